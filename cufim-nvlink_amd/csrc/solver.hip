@@ -1,0 +1,1185 @@
+// solver.hip -- gfx950 kernels of the FIM activation-time solve and the
+// device half of the C ABI (fim_create ... fim_destroy).
+//
+// Two modes share one pack (host.hpp):
+//
+//  converge (FP32, exact local solver): per global iteration one workgroup
+//    per active (instance, submesh) work item stages the submesh -- element
+//    nodes (4 x u16), metric geometry (6 x f32), vertex times (f32) -- in
+//    LDS and iterates Jacobi sweeps to local convergence.  The active
+//    element set is rebuilt every sweep from a vertex flag bitmask with
+//    wave64 ballots and a per-wave prefix (popcount); candidates are merged
+//    with LDS atomicMin on the float bit pattern (times are >= 0, so the
+//    IEEE order equals the unsigned order).  A second kernel pushes changed
+//    interface copies to the other copies of the vertex with global
+//    atomicMin, marks them dirty and appends their submesh to the next work
+//    list.  Both kernels read only their own work list, so results are
+//    independent of scheduling (atomic min and "set" flags commute).
+//    Iterations are captured in a hipGraph; the host reads one counter per
+//    graph launch.
+//
+//  compat (FP64, reference local update): one workgroup per submesh runs the
+//    reference's per-window phases a-g (src/eikonal/cuda_fim.cu:1992-2334)
+//    with vertex times and states in LDS; a separate kernel min-merges the
+//    copies of interface vertices (sync_step2, cuda_fim.cu:1752-1787).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "local_solve.hpp"
+
+#define HIPCHECK(x)                                                                   \
+  do {                                                                                \
+    hipError_t err_ = (x);                                                            \
+    if (err_ != hipSuccess)                                                           \
+      throw fim::Error(FIM_EDEVICE, std::string(#x) + ": " + hipGetErrorString(err_)); \
+  } while (0)
+
+namespace fim {
+namespace {
+
+constexpr float F_INF = __builtin_huge_valf();
+constexpr int kHardLocalCap = 4096;  // sweeps per activation before requeue
+
+__device__ inline uint64_t lanes_below() {
+  const uint32_t lane = threadIdx.x & 63;
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+__device__ inline bool flag_at(const uint32_t *bits, uint32_t l) {
+  return (bits[l >> 5] >> (l & 31)) & 1u;
+}
+
+// ---------------------------------------------------------------------------
+// converge mode
+// ---------------------------------------------------------------------------
+struct ConvArgs {
+  const SubInfo *sub;
+  const uint2 *e2n;      // 4 x u16 local ids per element
+  const float *geo;      // 6 per element
+  const uint8_t *ereg;   // region per element
+  const float *rscale;   // n_inst x 4 geometry multipliers (1/s^2)
+  float *phi;            // n_inst x ncopy
+  uint8_t *dirty;        // n_inst x ncopy
+  uint8_t *chg;          // n_inst x ncopy
+  uint32_t *act;         // n_inst x nsub
+  const uint32_t *list;  // current work list
+  const uint32_t *count;
+  uint32_t *next_list;   // next work list (requeue on the sweep cap)
+  uint32_t *next_count;  // filled by requeue here and by the sync kernel
+  uint32_t *zero_count;  // counter of the list after next: zeroed here.  It
+                         // was consumed by the previous iteration and is
+                         // appended to only from the next iteration on.
+  unsigned long long *stats;  // activations, sweeps, element evaluations, iterations
+  uint32_t nsub, ncopy;
+  float eps;
+  int32_t max_local;
+  // LDS carve (bytes)
+  uint32_t o_phin, o_flg, o_tch, o_e2n, o_geo, o_list, o_misc;
+};
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  float *phi_l = reinterpret_cast<float *>(lds);
+  uint32_t *phi_n = reinterpret_cast<uint32_t *>(lds + a.o_phin);
+  uint32_t *flg = reinterpret_cast<uint32_t *>(lds + a.o_flg);
+  uint32_t *tch = reinterpret_cast<uint32_t *>(lds + a.o_tch);
+  uint2 *el = reinterpret_cast<uint2 *>(lds + a.o_e2n);
+  float *gl = reinterpret_cast<float *>(lds + a.o_geo);
+  uint16_t *elist = reinterpret_cast<uint16_t *>(lds + a.o_list);
+  uint32_t *misc = reinterpret_cast<uint32_t *>(lds + a.o_misc);
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+  const uint32_t n_work = *a.count;
+  if (blockIdx.x == 0 && tid == 0) {
+    *a.zero_count = 0;
+    if (n_work > 0) atomicAdd(&a.stats[3], 1ull);
+  }
+  unsigned long long sweeps = 0, evals = 0, acts = 0;
+
+  for (uint32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
+    const uint32_t entry = a.list[w];
+    const uint32_t inst = entry / a.nsub, s = entry - inst * a.nsub;
+    const SubInfo si = a.sub[s];
+    const size_t cb = (size_t)inst * a.ncopy + si.coff;
+    const uint32_t nn = si.nn, ne = si.ne;
+    const uint32_t nv64 = (nn + 63u) & ~63u;
+    if (tid == 0) a.act[entry] = 0u;
+    // stage vertex times and dirty flags
+    for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
+      const uint32_t l = l0 + tid;
+      bool d = false;
+      if (l < nn) {
+        const float v = a.phi[cb + l];
+        phi_l[l] = v;
+        phi_n[l] = __float_as_uint(v);
+        d = a.dirty[cb + l] != 0;
+        if (d) a.dirty[cb + l] = 0;
+      }
+      const uint64_t m = __ballot(d);
+      const uint32_t wb = l0 + wave * 64;
+      if (wb < nv64) {
+        if (lane == 0) flg[wb >> 5] = (uint32_t)m;
+        if (lane == 1) flg[(wb >> 5) + 1] = (uint32_t)(m >> 32);
+        if (lane == 2) tch[wb >> 5] = 0u;
+        if (lane == 3) tch[(wb >> 5) + 1] = 0u;
+      }
+    }
+    // stage elements (geometry scaled by the instance's regional factor)
+    const float *rs = a.rscale + 4 * (size_t)inst;
+    for (uint32_t e = tid; e < ne; e += BS) {
+      const size_t ge = (size_t)si.eoff + e;
+      el[e] = a.e2n[ge];
+      const float sc = rs[a.ereg[ge] & 3];
+      const float2 *g2 = reinterpret_cast<const float2 *>(a.geo + 6 * ge);
+      const float2 x0 = g2[0], x1 = g2[1], x2 = g2[2];
+      float *o = gl + 6 * e;
+      o[0] = x0.x * sc;
+      o[1] = x0.y * sc;
+      o[2] = x1.x * sc;
+      o[3] = x1.y * sc;
+      o[4] = x2.x * sc;
+      o[5] = x2.y * sc;
+    }
+    __syncthreads();
+
+    int it = 0;
+    bool capped = false;
+    for (;;) {
+      if (tid == 0) misc[0] = 0u;
+      __syncthreads();
+      // active elements: any node flagged (wave ballot + per-wave base)
+      for (uint32_t e0 = 0; e0 < ne; e0 += BS) {
+        const uint32_t e = e0 + tid;
+        bool on = false;
+        if (e < ne) {
+          const uint2 q = el[e];
+          on = flag_at(flg, q.x & 0xffffu) | flag_at(flg, q.x >> 16) |
+               flag_at(flg, q.y & 0xffffu) | flag_at(flg, q.y >> 16);
+        }
+        const uint64_t m = __ballot(on);
+        uint32_t base = 0;
+        if (lane == 0 && m) base = atomicAdd(&misc[0], (uint32_t)__popcll(m));
+        base = __shfl(base, 0);
+        if (on) elist[base + __popcll(m & lanes_below())] = (uint16_t)e;
+      }
+      __syncthreads();
+      const uint32_t nact = misc[0];
+      if (nact == 0) break;
+      if ((a.max_local > 0 && it >= a.max_local) || it >= kHardLocalCap) {
+        capped = true;
+        break;
+      }
+      ++it;
+      evals += nact;
+      // candidates of the active elements
+      for (uint32_t i = tid; i < nact; i += BS) {
+        const uint32_t e = elist[i];
+        const uint2 q = el[e];
+        const uint32_t nd[4] = {q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16};
+        float g[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) g[k] = gl[6 * e + k];
+        float ph[4];
+        int f[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          ph[k] = phi_l[nd[k]];
+          f[k] = flag_at(flg, nd[k]) ? 1 : 0;
+        }
+        const int nf = f[0] + f[1] + f[2] + f[3];
+        // a target can improve only through a flagged other node and only
+        // if it is later than the earliest other node
+        if (nf - f[0] > 0 && ph[0] > fminf(ph[1], fminf(ph[2], ph[3]))) {
+          const float c = cand_exact<0, float>(g, ph, F_INF);
+          if (c < ph[0]) atomicMin(&phi_n[nd[0]], __float_as_uint(c));
+        }
+        if (nf - f[1] > 0 && ph[1] > fminf(ph[0], fminf(ph[2], ph[3]))) {
+          const float c = cand_exact<1, float>(g, ph, F_INF);
+          if (c < ph[1]) atomicMin(&phi_n[nd[1]], __float_as_uint(c));
+        }
+        if (nf - f[2] > 0 && ph[2] > fminf(ph[0], fminf(ph[1], ph[3]))) {
+          const float c = cand_exact<2, float>(g, ph, F_INF);
+          if (c < ph[2]) atomicMin(&phi_n[nd[2]], __float_as_uint(c));
+        }
+        if (nf - f[3] > 0 && ph[3] > fminf(ph[0], fminf(ph[1], ph[2]))) {
+          const float c = cand_exact<3, float>(g, ph, F_INF);
+          if (c < ph[3]) atomicMin(&phi_n[nd[3]], __float_as_uint(c));
+        }
+      }
+      __syncthreads();
+      // accept: new flags = significant decreases (reference criterion
+      // |old-new| >= eps (1 + (old+new)/2), cuda_fim.cu:2154)
+      for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
+        const uint32_t l = l0 + tid;
+        bool sig = false, dec = false;
+        if (l < nn) {
+          const float ov = phi_l[l];
+          const float nv = __uint_as_float(phi_n[l]);
+          if (nv < ov) {
+            phi_l[l] = nv;
+            dec = true;
+            sig = (ov - nv) >= a.eps * (1.0f + 0.5f * (ov + nv));
+          }
+        }
+        const uint64_t m = __ballot(sig), mt = __ballot(dec);
+        const uint32_t wb = l0 + wave * 64;
+        if (wb < nv64) {
+          if (lane == 0) flg[wb >> 5] = (uint32_t)m;
+          if (lane == 1) flg[(wb >> 5) + 1] = (uint32_t)(m >> 32);
+          if (lane == 2) tch[wb >> 5] |= (uint32_t)mt;
+          if (lane == 3) tch[(wb >> 5) + 1] |= (uint32_t)(mt >> 32);
+        }
+      }
+      __syncthreads();
+    }
+    sweeps += it;
+    acts += 1;
+    // write back decreased copies; remember them for the interface push
+    for (uint32_t l = tid; l < nn; l += BS) {
+      if (flag_at(tch, l)) {
+        a.phi[cb + l] = phi_l[l];
+        a.chg[cb + l] = 1;
+      }
+      if (capped && flag_at(flg, l)) a.dirty[cb + l] = 1;
+    }
+    if (capped && tid == 0) {
+      if (atomicExch(&a.act[entry], 1u) == 0u) {
+        const uint32_t pos = atomicAdd(a.next_count, 1u);
+        a.next_list[pos] = entry;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && acts) {
+    atomicAdd(&a.stats[0], acts);
+    atomicAdd(&a.stats[1], sweeps);
+    atomicAdd(&a.stats[2], evals);
+  }
+}
+
+struct SyncArgs {
+  const SubInfo *sub;
+  const uint16_t *if_l;
+  const uint32_t *if_dsp;
+  const uint32_t *if_other;
+  const uint32_t *c2s;
+  float *phi;
+  uint8_t *dirty;
+  uint8_t *chg;
+  uint32_t *act;
+  const uint32_t *list;
+  const uint32_t *count;
+  uint32_t *next_list;
+  uint32_t *next_count;
+  uint32_t nsub, ncopy;
+};
+
+// interface push: min over copies, order independent
+__global__ __launch_bounds__(256) void k_conv_sync(SyncArgs a) {
+  const uint32_t n_work = *a.count;
+  for (uint32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
+    const uint32_t entry = a.list[w];
+    const uint32_t inst = entry / a.nsub, s = entry - inst * a.nsub;
+    const SubInfo si = a.sub[s];
+    const size_t ib = (size_t)inst * a.ncopy;
+    for (uint32_t j = threadIdx.x; j < si.ni; j += blockDim.x) {
+      const uint32_t idx = si.ioff + j;
+      const size_t gc = ib + si.coff + a.if_l[idx];
+      if (!a.chg[gc]) continue;
+      a.chg[gc] = 0;
+      const uint32_t v = __float_as_uint(a.phi[gc]);
+      for (uint32_t k = a.if_dsp[idx]; k < a.if_dsp[idx + 1]; ++k) {
+        const uint32_t oc = a.if_other[k];
+        uint32_t *p = reinterpret_cast<uint32_t *>(a.phi + ib + oc);
+        if (v < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          const uint32_t old = atomicMin(p, v);
+          if (v < old) {
+            a.dirty[ib + oc] = 1;
+            const uint32_t e2 = inst * a.nsub + a.c2s[oc];
+            if (atomicExch(&a.act[e2], 1u) == 0u) {
+              const uint32_t pos = atomicAdd(a.next_count, 1u);
+              a.next_list[pos] = e2;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// seeds: copies (within one instance) get min(phi, t0), are flagged, and
+// their submesh is queued on the current list
+__global__ void k_conv_seed(const uint32_t *copies, uint32_t n, uint32_t inst, float t0,
+                            const uint32_t *c2s, float *phi, uint8_t *dirty, uint32_t *act,
+                            uint32_t *list, uint32_t *count, uint32_t nsub, uint32_t ncopy) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t c = copies[i];
+    const size_t gc = (size_t)inst * ncopy + c;
+    atomicMin(reinterpret_cast<uint32_t *>(phi + gc), __float_as_uint(t0));
+    dirty[gc] = 1;
+    const uint32_t e = inst * nsub + c2s[c];
+    if (atomicExch(&act[e], 1u) == 0u) list[atomicAdd(count, 1u)] = e;
+  }
+}
+
+// warm start: every copy of the instance takes the global field value,
+// everything flagged, every submesh queued
+__global__ void k_conv_init(const float *phi0, const uint32_t *c2g, uint32_t inst, float *phi,
+                            uint8_t *dirty, uint32_t ncopy) {
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncopy; c += gridDim.x * blockDim.x) {
+    const size_t gc = (size_t)inst * ncopy + c;
+    phi[gc] = phi0[c2g[c]];
+    dirty[gc] = 1;
+  }
+}
+
+__global__ void k_conv_queue_all(uint32_t inst, uint32_t nsub, uint32_t *act, uint32_t *list,
+                                 uint32_t *count) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nsub; s += gridDim.x * blockDim.x) {
+    const uint32_t e = inst * nsub + s;
+    if (atomicExch(&act[e], 1u) == 0u) list[atomicAdd(count, 1u)] = e;
+  }
+}
+
+template <class T>
+__global__ void k_fill(T *p, size_t n, T v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// min over copies of each global vertex
+__global__ void k_conv_gather(const float *phi, const uint32_t *g2c_dsp, const uint32_t *g2c,
+                              uint32_t nn, size_t ib, float *out) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nn; v += gridDim.x * blockDim.x) {
+    float m = F_INF;
+    for (uint32_t k = g2c_dsp[v]; k < g2c_dsp[v + 1]; ++k) m = fminf(m, phi[ib + g2c[k]]);
+    out[v] = m;
+  }
+}
+
+// residual sweep: max relative amount any copy could still be lowered
+__global__ __launch_bounds__(256) void k_conv_residual(const SubInfo *sub, uint32_t nsub,
+                                                       const uint2 *e2n, const float *geo,
+                                                       const uint8_t *ereg, const float *rscale,
+                                                       const float *phi, size_t ib, uint32_t inst,
+                                                       uint32_t *res_bits) {
+  float worst = 0.0f;
+  for (uint32_t s = blockIdx.x; s < nsub; s += gridDim.x) {
+    const SubInfo si = sub[s];
+    for (uint32_t e = threadIdx.x; e < si.ne; e += blockDim.x) {
+      const size_t ge = (size_t)si.eoff + e;
+      const uint2 q = e2n[ge];
+      const uint32_t nd[4] = {q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16};
+      const float sc = rscale[4 * (size_t)inst + (ereg[ge] & 3)];
+      float g[6], ph[4];
+      for (int k = 0; k < 6; ++k) g[k] = geo[6 * ge + k] * sc;
+      for (int k = 0; k < 4; ++k) ph[k] = phi[ib + si.coff + nd[k]];
+      float c[4] = {cand_exact<0, float>(g, ph, F_INF), cand_exact<1, float>(g, ph, F_INF),
+                    cand_exact<2, float>(g, ph, F_INF), cand_exact<3, float>(g, ph, F_INF)};
+      for (int k = 0; k < 4; ++k)
+        if (c[k] < ph[k]) worst = fmaxf(worst, ph[k] == F_INF ? F_INF : (ph[k] - c[k]) / (1.0f + ph[k]));
+    }
+  }
+  if (worst > 0.0f) atomicMax(res_bits, __float_as_uint(worst));
+}
+
+// ---------------------------------------------------------------------------
+// compat mode (reference time windows, FP64)
+// ---------------------------------------------------------------------------
+struct CompatArgs {
+  const SubInfo *sub;
+  const uint2 *e2n;
+  const double *geo;
+  const uint32_t *n2e_dsp;
+  const uint16_t *n2e_le;
+  const uint8_t *n2e_pos;
+  const uint32_t *n2n_dsp;
+  const uint16_t *n2n_lv;
+  double *phi;
+  int8_t *st;
+  unsigned long long *stats;  // sweeps, slot evaluations
+  uint32_t *pending;          // copies left narrow/next after the window
+  uint32_t nsub;
+  int32_t exact;
+  double fw, cur, phi_start, dt, apd;
+  uint32_t o_tphi, o_st, o_tst, o_snap, o_misc;
+};
+
+__device__ inline uint32_t block_sum(uint32_t v, uint32_t *slot) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if (threadIdx.x == 0) *slot = 0u;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) atomicAdd(slot, v);
+  __syncthreads();
+  const uint32_t r = *slot;
+  __syncthreads();
+  return r;
+}
+
+// one (vertex, element) slot: src/eikonal/cuda_fim.h:1362-1411
+__device__ inline double compat_slot(const CompatArgs &a, const SubInfo &si, const double *phi_l,
+                                     const int8_t *st_l, uint32_t k) {
+  const uint32_t le = a.n2e_le[k];
+  const int t = a.n2e_pos[k];
+  const size_t ge = (size_t)si.eoff + le;
+  const uint2 q = a.e2n[ge];
+  const uint32_t nd[4] = {q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16};
+  double ph[4], g[6];
+  const double lo = a.phi_start - a.fw;
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t n = nd[j];
+    ph[j] = (j != t && st_l[n] >= FIM_NARROW && phi_l[n] > lo) ? phi_l[n] : DBL_MAX;
+  }
+  for (int j = 0; j < 6; ++j) g[j] = a.geo[6 * ge + j];
+  if (!a.exact) return cand_ref(g, ph, t);
+  switch (t) {
+    case 0: return cand_exact<0, double>(g, ph, DBL_MAX);
+    case 1: return cand_exact<1, double>(g, ph, DBL_MAX);
+    case 2: return cand_exact<2, double>(g, ph, DBL_MAX);
+    default: return cand_exact<3, double>(g, ph, DBL_MAX);
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_compat_compute(CompatArgs a) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char lds[];
+  double *phi_l = reinterpret_cast<double *>(lds);
+  double *tphi = reinterpret_cast<double *>(lds + a.o_tphi);
+  int8_t *st_l = reinterpret_cast<int8_t *>(lds + a.o_st);
+  int8_t *tst = reinterpret_cast<int8_t *>(lds + a.o_tst);
+  int8_t *snap = reinterpret_cast<int8_t *>(lds + a.o_snap);
+  uint32_t *misc = reinterpret_cast<uint32_t *>(lds + a.o_misc);
+  const uint32_t tid = threadIdx.x;
+  unsigned long long sweeps = 0, slots = 0;
+  for (uint32_t s = blockIdx.x; s < a.nsub; s += gridDim.x) {
+    const SubInfo si = a.sub[s];
+    const uint32_t nn = si.nn;
+    // update_states (cuda_fim.cu:1790-1845), applied to every submesh
+    uint32_t cnt = 0;
+    for (uint32_t l = tid; l < nn; l += BS) {
+      double p = a.phi[si.coff + l];
+      int8_t st = a.st[si.coff + l];
+      if (st == FIM_NARROW) {
+        ++cnt;
+      } else if (st == FIM_NEXT) {
+        ++cnt;
+        st = FIM_NARROW;
+      } else if (st == FIM_FROZEN) {
+        if (!(p + a.apd > a.cur)) st = FIM_TAIL;
+      } else if (st == FIM_TAIL) {
+        if (!(p + a.apd / 0.91 > a.cur)) {
+          p = DBL_MAX;
+          st = FIM_FAR;
+        }
+      }
+      phi_l[l] = p;
+      st_l[l] = st;
+    }
+    uint32_t active = block_sum(cnt, misc);
+    while (active > 0) {
+      ++sweeps;
+      // a: candidates of narrow vertices
+      for (uint32_t l = tid; l < nn; l += BS) {
+        double m = DBL_MAX;
+        if (st_l[l] == FIM_NARROW) {
+          const uint32_t c = si.coff + l;
+          for (uint32_t k = a.n2e_dsp[c]; k < a.n2e_dsp[c + 1]; ++k) {
+            const double v = compat_slot(a, si, phi_l, st_l, k);
+            if (v < m) m = v;
+            ++slots;
+          }
+        }
+        tphi[l] = m;
+      }
+      __syncthreads();
+      // b: reduce, window test, convergence test
+      for (uint32_t l = tid; l < nn; l += BS) {
+        tst[l] = FIM_FAR;
+        if (st_l[l] != FIM_NARROW) continue;
+        const double old = phi_l[l];
+        const double nw = tphi[l] < old ? tphi[l] : old;
+        phi_l[l] = nw;
+        if (nw - a.phi_start > a.dt)
+          st_l[l] = FIM_NEXT;
+        else if (fabs(old - nw) < (1e-6 * (1 + (old + nw) * .5)))
+          tst[l] = FIM_FROZEN;
+      }
+      __syncthreads();
+      // c: downwind neighbours of just-frozen vertices (flags as after b)
+      for (uint32_t l = tid; l < nn; l += BS) snap[l] = tst[l];
+      __syncthreads();
+      for (uint32_t l = tid; l < nn; l += BS) {
+        if (st_l[l] == FIM_NARROW) continue;
+        const uint32_t c = si.coff + l;
+        for (uint32_t k = a.n2n_dsp[c]; k < a.n2n_dsp[c + 1]; ++k) {
+          const uint32_t n = a.n2n_lv[k];
+          if (snap[n] == FIM_FROZEN && phi_l[n] < phi_l[l]) {
+            tst[l] = st_l[l];
+            st_l[l] = FIM_NEIG;
+            break;
+          }
+        }
+      }
+      __syncthreads();
+      // d: candidates of marked vertices
+      for (uint32_t l = tid; l < nn; l += BS) {
+        if (st_l[l] != FIM_NEIG) continue;
+        double m = DBL_MAX;
+        const uint32_t c = si.coff + l;
+        for (uint32_t k = a.n2e_dsp[c]; k < a.n2e_dsp[c + 1]; ++k) {
+          const double v = compat_slot(a, si, phi_l, st_l, k);
+          if (v < m) m = v;
+          ++slots;
+        }
+        tphi[l] = m;
+      }
+      __syncthreads();
+      // e: accept improvements
+      for (uint32_t l = tid; l < nn; l += BS) {
+        if (st_l[l] != FIM_NEIG) continue;
+        const double old = phi_l[l];
+        const double nw = tphi[l] < old ? tphi[l] : old;
+        phi_l[l] = nw;
+        st_l[l] = (nw < old) ? (int8_t)FIM_NARROW : tst[l];
+      }
+      __syncthreads();
+      // f: freeze / count
+      cnt = 0;
+      for (uint32_t l = tid; l < nn; l += BS) {
+        if (st_l[l] == FIM_NARROW) {
+          if (tst[l] == FIM_FROZEN)
+            st_l[l] = FIM_FROZEN;
+          else
+            ++cnt;
+        } else if (st_l[l] == FIM_NEIG) {
+          st_l[l] = FIM_NARROW;
+          ++cnt;
+        }
+      }
+      active = block_sum(cnt, misc);
+    }
+    uint32_t pend = 0;
+    for (uint32_t l = tid; l < nn; l += BS) {
+      a.phi[si.coff + l] = phi_l[l];
+      a.st[si.coff + l] = st_l[l];
+      pend += (st_l[l] == FIM_NARROW || st_l[l] == FIM_NEXT);
+    }
+    pend = block_sum(pend, misc);
+    if (tid == 0 && pend) atomicAdd(a.pending, pend);
+  }
+  if (tid == 0 && sweeps) atomicAdd(&a.stats[0], sweeps);
+  for (int off = 32; off > 0; off >>= 1) slots += __shfl_xor(slots, off);
+  if ((tid & 63) == 0 && slots) atomicAdd(&a.stats[1], slots);
+}
+
+// sync_step2 over vertices with several copies
+__global__ void k_compat_sync(const uint32_t *grp_v, uint32_t ngrp, const uint32_t *g2c_dsp,
+                              const uint32_t *g2c, double *phi, int8_t *st) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ngrp; i += gridDim.x * blockDim.x) {
+    const uint32_t v = grp_v[i];
+    double mn = DBL_MAX;
+    for (uint32_t k = g2c_dsp[v]; k < g2c_dsp[v + 1]; ++k) {
+      const double c = phi[g2c[k]];
+      if (c < mn) mn = c;
+    }
+    for (uint32_t k = g2c_dsp[v]; k < g2c_dsp[v + 1]; ++k) {
+      const uint32_t c = g2c[k];
+      if (mn < phi[c]) {
+        phi[c] = mn;
+        st[c] = FIM_NARROW;
+      }
+    }
+  }
+}
+
+__global__ void k_compat_seed(const uint32_t *copies, uint32_t n, double t0, double *phi, int8_t *st) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    phi[copies[i]] = t0;
+    st[copies[i]] = FIM_NARROW;
+  }
+}
+
+__global__ void k_compat_gather(const double *phi, const int8_t *st, const uint32_t *g2c_dsp,
+                                const uint32_t *g2c, uint32_t nn, double *out_phi, int32_t *out_st) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nn; v += gridDim.x * blockDim.x) {
+    double m = DBL_MAX;
+    int32_t s = FIM_FAR;
+    for (uint32_t k = g2c_dsp[v]; k < g2c_dsp[v + 1]; ++k) {
+      const uint32_t c = g2c[k];
+      if (phi[c] < m) m = phi[c];
+      if (st[c] > s) s = st[c];
+    }
+    out_phi[v] = m;
+    out_st[v] = s;
+  }
+}
+
+uint32_t al16(uint32_t x) { return (x + 15u) & ~15u; }
+
+template <class T>
+T *dalloc(size_t n, std::vector<void *> &owned) {
+  void *p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHECK(hipMalloc(&p, n * sizeof(T)));
+  owned.push_back(p);
+  return static_cast<T *>(p);
+}
+
+template <class T>
+T *dupload(const std::vector<T> &v, std::vector<void *> &owned) {
+  T *p = dalloc<T>(v.size(), owned);
+  if (!v.empty()) HIPCHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+}  // namespace
+}  // namespace fim
+
+using namespace fim;
+
+struct fim_ctx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  fim_opts o{};
+  uint32_t nsub = 0, ncopy = 0, ninst = 1, nn = 0;
+  int64_t ne = 0;
+  int32_t nn_max = 0, ne_max = 0;
+  std::vector<void *> owned;
+  // host copies for seeding
+  std::vector<uint32_t> g2c_dsp, g2c;
+  // pack
+  SubInfo *d_sub = nullptr;
+  uint2 *d_e2n = nullptr;
+  float *d_geo32 = nullptr;
+  double *d_geo64 = nullptr;
+  uint8_t *d_ereg = nullptr;
+  uint32_t *d_c2g = nullptr, *d_c2s = nullptr, *d_g2c_dsp = nullptr, *d_g2c = nullptr, *d_grp = nullptr;
+  uint32_t ngrp = 0;
+  uint16_t *d_if_l = nullptr;
+  uint32_t *d_if_dsp = nullptr, *d_if_other = nullptr;
+  uint32_t *d_n2e_dsp = nullptr, *d_n2n_dsp = nullptr;
+  uint16_t *d_n2e_le = nullptr, *d_n2n_lv = nullptr;
+  uint8_t *d_n2e_pos = nullptr;
+  // converge state
+  float *d_phi = nullptr, *d_rscale = nullptr;
+  uint8_t *d_dirty = nullptr, *d_chg = nullptr;
+  uint32_t *d_act = nullptr, *d_list[3] = {nullptr, nullptr, nullptr}, *d_cnt = nullptr;
+  // compat state
+  double *d_phi64 = nullptr;
+  int8_t *d_st = nullptr;
+  uint32_t *d_pending = nullptr;
+  // shared
+  unsigned long long *d_stats = nullptr;
+  uint32_t *h_pin = nullptr;
+  float *d_tmpf = nullptr;
+  double *d_tmpd = nullptr;
+  int32_t *d_tmpi = nullptr;
+  uint32_t *d_tmpu = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  ConvArgs ca{};
+  SyncArgs sa{};
+  CompatArgs xa{};
+  uint32_t lds_conv = 0, lds_compat = 0;
+  int grid = 0, block = 256, graph_iters = 8;
+  fim_stats_t stats{};
+};
+
+namespace {
+
+void launch_conv_compute(fim_ctx *c, const ConvArgs &a) {
+  switch (c->block) {
+    case 64: k_conv_compute<64><<<c->grid, 64, c->lds_conv, c->stream>>>(a); break;
+    case 128: k_conv_compute<128><<<c->grid, 128, c->lds_conv, c->stream>>>(a); break;
+    case 512: k_conv_compute<512><<<c->grid, 512, c->lds_conv, c->stream>>>(a); break;
+    default: k_conv_compute<256><<<c->grid, 256, c->lds_conv, c->stream>>>(a); break;
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+void set_lds_attr(fim_ctx *c) {
+  const void *f = nullptr;
+  switch (c->block) {
+    case 64: f = (const void *)k_conv_compute<64>; break;
+    case 128: f = (const void *)k_conv_compute<128>; break;
+    case 512: f = (const void *)k_conv_compute<512>; break;
+    default: f = (const void *)k_conv_compute<256>; break;
+  }
+  HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_conv));
+  const void *g = nullptr;
+  switch (c->block) {
+    case 64: g = (const void *)k_compat_compute<64>; break;
+    case 128: g = (const void *)k_compat_compute<128>; break;
+    case 512: g = (const void *)k_compat_compute<512>; break;
+    default: g = (const void *)k_compat_compute<256>; break;
+  }
+  HIPCHECK(hipFuncSetAttribute(g, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_compat));
+}
+
+void launch_compat_compute(fim_ctx *c, const CompatArgs &a) {
+  const int grid = (int)std::min<uint32_t>(c->nsub, 4096u);
+  switch (c->block) {
+    case 64: k_compat_compute<64><<<grid, 64, c->lds_compat, c->stream>>>(a); break;
+    case 128: k_compat_compute<128><<<grid, 128, c->lds_compat, c->stream>>>(a); break;
+    case 512: k_compat_compute<512><<<grid, 512, c->lds_compat, c->stream>>>(a); break;
+    default: k_compat_compute<256><<<grid, 256, c->lds_compat, c->stream>>>(a); break;
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+int blocks_for(size_t n, int bs) { return (int)std::max<size_t>(1, std::min<size_t>((n + bs - 1) / bs, 65536)); }
+
+// one global iteration k: work lists rotate over three buffers; iteration k
+// reads list k%3, appends to (k+1)%3 and zeroes the counter of (k+2)%3
+void enqueue_iteration(fim_ctx *c, int k) {
+  const int p = k % 3, q = (k + 1) % 3, z = (k + 2) % 3;
+  ConvArgs a = c->ca;
+  a.list = c->d_list[p];
+  a.count = c->d_cnt + p;
+  a.next_list = c->d_list[q];
+  a.next_count = c->d_cnt + q;
+  a.zero_count = c->d_cnt + z;
+  launch_conv_compute(c, a);
+  SyncArgs s = c->sa;
+  s.list = c->d_list[p];
+  s.count = c->d_cnt + p;
+  s.next_list = c->d_list[q];
+  s.next_count = c->d_cnt + q;
+  k_conv_sync<<<c->grid, 256, 0, c->stream>>>(s);
+  HIPCHECK(hipGetLastError());
+}
+
+void build_graph(fim_ctx *c) {
+  HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int i = 0; i < c->graph_iters; ++i) enqueue_iteration(c, i);
+  } catch (...) {
+    hipGraph_t g;
+    (void)hipStreamEndCapture(c->stream, &g);
+    throw;
+  }
+  HIPCHECK(hipStreamEndCapture(c->stream, &c->graph));
+  HIPCHECK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+}
+
+void check_inst(const fim_ctx *c, int32_t inst) {
+  if (inst < 0 || (uint32_t)inst >= c->ninst) throw Error(FIM_EINVAL, "instance index out of range");
+}
+
+std::vector<uint32_t> copies_of(const fim_ctx *c, const int64_t *vtx, int64_t n) {
+  std::vector<uint32_t> out;
+  for (int64_t i = 0; i < n; ++i) {
+    if (vtx[i] < 0 || vtx[i] >= (int64_t)c->nn) throw Error(FIM_EINVAL, "seed vertex out of range");
+    for (uint32_t k = c->g2c_dsp[vtx[i]]; k < c->g2c_dsp[vtx[i] + 1]; ++k) out.push_back(c->g2c[k]);
+  }
+  return out;
+}
+
+void destroy_ctx(fim_ctx *c) {
+  if (!c) return;
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (void *p : c->owned) (void)hipFree(p);
+  if (c->h_pin) (void)hipHostFree(c->h_pin);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return FIM_OK;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return FIM_EINVAL;
+  } catch (...) {
+    set_last_error("unknown error");
+    return FIM_EINVAL;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
+  return guarded([&] {
+    if (!pk || !o || !out) throw Error(FIM_EINVAL, "null argument");
+    *out = nullptr;
+    const Pack &p = pk->p;
+    fim_ctx *c = new fim_ctx();
+    try {
+      c->o = *o;
+      c->dev = o->device;
+      HIPCHECK(hipSetDevice(c->dev));
+      HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      HIPCHECK(hipEventCreate(&c->ev0));
+      HIPCHECK(hipEventCreate(&c->ev1));
+      if (o->mode != FIM_MODE_CONVERGE && o->mode != FIM_MODE_COMPAT)
+        throw Error(FIM_EINVAL, "unknown mode");
+      c->block = o->block ? o->block : 256;
+      if (c->block != 64 && c->block != 128 && c->block != 256 && c->block != 512)
+        throw Error(FIM_EINVAL, "block must be 64, 128, 256 or 512");
+      c->ninst = o->mode == FIM_MODE_CONVERGE ? (uint32_t)std::max(1, o->n_instances) : 1u;
+      // a multiple of 3 so every graph launch starts on list 0
+      c->graph_iters = o->graph_iters > 0 ? 3 * ((o->graph_iters + 2) / 3) : 6;
+      c->nsub = (uint32_t)p.nsub;
+      c->ncopy = (uint32_t)p.ncopy;
+      c->nn = (uint32_t)p.nn_global;
+      c->ne = p.ne_global;
+      c->nn_max = p.nn_max;
+      c->ne_max = p.ne_max;
+      if ((uint64_t)c->ninst * c->nsub >= (1ull << 32) || (uint64_t)c->ninst * c->ncopy >= (1ull << 40))
+        throw Error(FIM_ELIMIT, "too many instances for this mesh");
+      c->g2c_dsp = p.g2c_dsp;
+      c->g2c = p.g2c;
+      auto &ow = c->owned;
+      c->d_sub = dupload(p.sub, ow);
+      {
+        std::vector<uint2> e2(p.ne_global);
+        for (int64_t e = 0; e < p.ne_global; ++e) {
+          e2[e].x = (uint32_t)p.e2n_l[4 * e] | ((uint32_t)p.e2n_l[4 * e + 1] << 16);
+          e2[e].y = (uint32_t)p.e2n_l[4 * e + 2] | ((uint32_t)p.e2n_l[4 * e + 3] << 16);
+        }
+        c->d_e2n = dupload(e2, ow);
+      }
+      c->d_ereg = dupload(p.ereg, ow);
+      c->d_c2g = dupload(p.c2g, ow);
+      c->d_c2s = dupload(p.c2s, ow);
+      c->d_g2c_dsp = dupload(p.g2c_dsp, ow);
+      c->d_g2c = dupload(p.g2c, ow);
+      c->d_stats = dalloc<unsigned long long>(8, ow);
+      HIPCHECK(hipHostMalloc((void **)&c->h_pin, 64, hipHostMallocDefault));
+      c->d_tmpu = dalloc<uint32_t>(4, ow);
+      HIPCHECK(hipMemset(c->d_stats, 0, 8 * sizeof(unsigned long long)));
+      if (o->mode == FIM_MODE_CONVERGE) {
+        c->lds_conv = 0;
+        const uint32_t nn64 = ((uint32_t)c->nn_max + 63u) & ~63u;
+        const uint32_t o_phin = al16(4 * nn64);
+        const uint32_t o_flg = o_phin + al16(4 * nn64);
+        const uint32_t o_tch = o_flg + al16(4 * (nn64 / 32));
+        const uint32_t o_e2n = o_tch + al16(4 * (nn64 / 32));
+        const uint32_t o_geo = o_e2n + al16(8 * (uint32_t)c->ne_max);
+        const uint32_t o_list = o_geo + al16(24 * (uint32_t)c->ne_max);
+        const uint32_t o_misc = o_list + al16(2 * (uint32_t)c->ne_max);
+        c->lds_conv = o_misc + 16;
+        if (c->lds_conv > 160 * 1024)
+          throw Error(FIM_ELIMIT, "largest submesh needs " + std::to_string(c->lds_conv) +
+                                      " B of LDS (limit 163840); use a smaller partition budget");
+        c->d_geo32 = dupload(p.geo32, ow);
+        c->d_if_l = dupload(p.if_l, ow);
+        c->d_if_dsp = dupload(p.if_dsp, ow);
+        c->d_if_other = dupload(p.if_other, ow);
+        const size_t ncs = (size_t)c->ninst * c->ncopy, nss = (size_t)c->ninst * c->nsub;
+        c->d_phi = dalloc<float>(ncs, ow);
+        c->d_dirty = dalloc<uint8_t>(ncs, ow);
+        c->d_chg = dalloc<uint8_t>(ncs, ow);
+        c->d_act = dalloc<uint32_t>(nss, ow);
+        for (int k = 0; k < 3; ++k) c->d_list[k] = dalloc<uint32_t>(nss, ow);
+        c->d_cnt = dalloc<uint32_t>(3, ow);
+        c->d_rscale = dalloc<float>(4 * (size_t)c->ninst, ow);
+        c->d_tmpf = dalloc<float>(c->nn, ow);
+        k_fill<float><<<blocks_for(4 * (size_t)c->ninst, 256), 256, 0, c->stream>>>(c->d_rscale, 4 * (size_t)c->ninst, 1.0f);
+        int ncu = 256;
+        HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
+        const int per_cu = std::max(1, (int)((160 * 1024) / std::max<uint32_t>(c->lds_conv, 1)));
+        c->grid = o->grid > 0 ? o->grid : ncu * std::min(per_cu, 2048 / c->block);
+        c->lds_compat = 0;
+        c->ca = ConvArgs{c->d_sub, c->d_e2n, c->d_geo32, c->d_ereg, c->d_rscale, c->d_phi,
+                         c->d_dirty, c->d_chg, c->d_act, nullptr, nullptr, nullptr, nullptr,
+                         nullptr, c->d_stats, c->nsub, c->ncopy,
+                         (float)(o->eps_rel > 0 ? o->eps_rel : 1e-6), o->local_iters,
+                         o_phin, o_flg, o_tch, o_e2n, o_geo, o_list, o_misc};
+        c->sa = SyncArgs{c->d_sub, c->d_if_l, c->d_if_dsp, c->d_if_other, c->d_c2s, c->d_phi,
+                         c->d_dirty, c->d_chg, c->d_act, nullptr, nullptr, nullptr, nullptr,
+                         c->nsub, c->ncopy};
+      } else {
+        const uint32_t nn = (uint32_t)c->nn_max;
+        const uint32_t o_tphi = al16(8 * nn);
+        const uint32_t o_st = o_tphi + al16(8 * nn);
+        const uint32_t o_tst = o_st + al16(nn);
+        const uint32_t o_snap = o_tst + al16(nn);
+        const uint32_t o_misc = o_snap + al16(nn);
+        c->lds_compat = o_misc + 16;
+        if (c->lds_compat > 160 * 1024)
+          throw Error(FIM_ELIMIT, "largest submesh too large for compat mode LDS");
+        c->d_geo64 = dupload(p.geo64, ow);
+        c->d_grp = dupload(p.grp_v, ow);
+        c->ngrp = (uint32_t)p.grp_v.size();
+        c->d_n2e_dsp = dupload(p.n2e_dsp, ow);
+        c->d_n2e_le = dupload(p.n2e_le, ow);
+        c->d_n2e_pos = dupload(p.n2e_pos, ow);
+        c->d_n2n_dsp = dupload(p.n2n_dsp, ow);
+        c->d_n2n_lv = dupload(p.n2n_lv, ow);
+        c->d_phi64 = dalloc<double>(c->ncopy, ow);
+        c->d_st = dalloc<int8_t>(c->ncopy, ow);
+        c->d_pending = dalloc<uint32_t>(1, ow);
+        c->d_tmpd = dalloc<double>(c->nn, ow);
+        c->d_tmpi = dalloc<int32_t>(c->nn, ow);
+        c->lds_conv = 0;
+        c->xa = CompatArgs{c->d_sub, c->d_e2n, c->d_geo64, c->d_n2e_dsp, c->d_n2e_le, c->d_n2e_pos,
+                           c->d_n2n_dsp, c->d_n2n_lv, c->d_phi64, c->d_st, c->d_stats,
+                           c->d_pending, c->nsub, o->local == FIM_LOCAL_EXACT ? 1 : 0,
+                           0, 0, 0, 0, o->apd > 0 ? o->apd : 3e5,
+                           o_tphi, o_st, o_tst, o_snap, o_misc};
+      }
+      set_lds_attr(c);
+      HIPCHECK(hipStreamSynchronize(c->stream));
+      *out = c;
+    } catch (...) {
+      destroy_ctx(c);
+      throw;
+    }
+    if (fim_reset(*out) != FIM_OK) {
+      const std::string msg = fim_last_error();
+      destroy_ctx(*out);
+      *out = nullptr;
+      throw Error(FIM_EDEVICE, msg);
+    }
+  });
+}
+
+int fim_reset(fim_ctx *c) {
+  return guarded([&] {
+    if (!c) throw Error(FIM_EINVAL, "null context");
+    HIPCHECK(hipSetDevice(c->dev));
+    if (c->o.mode == FIM_MODE_CONVERGE) {
+      const size_t ncs = (size_t)c->ninst * c->ncopy, nss = (size_t)c->ninst * c->nsub;
+      k_fill<float><<<blocks_for(ncs, 256), 256, 0, c->stream>>>(c->d_phi, ncs, F_INF);
+      k_fill<uint8_t><<<blocks_for(ncs, 256), 256, 0, c->stream>>>(c->d_dirty, ncs, (uint8_t)0);
+      k_fill<uint8_t><<<blocks_for(ncs, 256), 256, 0, c->stream>>>(c->d_chg, ncs, (uint8_t)0);
+      k_fill<uint32_t><<<blocks_for(nss, 256), 256, 0, c->stream>>>(c->d_act, nss, 0u);
+      k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_cnt, 3, 0u);
+    } else {
+      k_fill<double><<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_phi64, c->ncopy, DBL_MAX);
+      k_fill<int8_t><<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_st, c->ncopy, (int8_t)FIM_FAR);
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int fim_seed(fim_ctx *c, int32_t inst, const int64_t *vtx, int64_t n, double t0) {
+  return guarded([&] {
+    if (!c || (!vtx && n > 0) || n < 0) throw Error(FIM_EINVAL, "bad seed arguments");
+    check_inst(c, inst);
+    if (n == 0) return;
+    HIPCHECK(hipSetDevice(c->dev));
+    std::vector<uint32_t> cp = copies_of(c, vtx, n);
+    uint32_t *d = nullptr;
+    HIPCHECK(hipMalloc(&d, cp.size() * sizeof(uint32_t)));
+    try {
+      HIPCHECK(hipMemcpyAsync(d, cp.data(), cp.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+      const int g = blocks_for(cp.size(), 256);
+      if (c->o.mode == FIM_MODE_CONVERGE) {
+        if (!(t0 >= 0.0)) throw Error(FIM_EINVAL, "converge mode needs t0 >= 0");
+        k_conv_seed<<<g, 256, 0, c->stream>>>(d, (uint32_t)cp.size(), (uint32_t)inst, (float)t0,
+                                              c->d_c2s, c->d_phi, c->d_dirty, c->d_act,
+                                              c->d_list[0], c->d_cnt, c->nsub, c->ncopy);
+      } else {
+        k_compat_seed<<<g, 256, 0, c->stream>>>(d, (uint32_t)cp.size(), t0, c->d_phi64, c->d_st);
+      }
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipStreamSynchronize(c->stream));
+    } catch (...) {
+      (void)hipFree(d);
+      throw;
+    }
+    HIPCHECK(hipFree(d));
+  });
+}
+
+int fim_set_scaling(fim_ctx *c, int32_t inst, const float scale[4]) {
+  return guarded([&] {
+    if (!c || !scale) throw Error(FIM_EINVAL, "null argument");
+    if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "scaling needs converge mode");
+    check_inst(c, inst);
+    float m[4];
+    for (int k = 0; k < 4; ++k) {
+      if (!(scale[k] > 0.0f)) throw Error(FIM_EINVAL, "scalings must be positive");
+      m[k] = 1.0f / (scale[k] * scale[k]);  // both speeds scaled: M scales by 1/s^2
+    }
+    HIPCHECK(hipSetDevice(c->dev));
+    HIPCHECK(hipMemcpyAsync(c->d_rscale + 4 * (size_t)inst, m, sizeof(m), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int fim_init_field(fim_ctx *c, int32_t inst, const float *phi0) {
+  return guarded([&] {
+    if (!c || !phi0) throw Error(FIM_EINVAL, "null argument");
+    if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "warm start needs converge mode");
+    check_inst(c, inst);
+    for (uint32_t v = 0; v < c->nn; ++v)
+      if (!(phi0[v] >= 0.0f)) throw Error(FIM_EINVAL, "initial field must be >= 0 (inf allowed)");
+    HIPCHECK(hipSetDevice(c->dev));
+    HIPCHECK(hipMemcpyAsync(c->d_tmpf, phi0, c->nn * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    k_conv_init<<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_tmpf, c->d_c2g, (uint32_t)inst,
+                                                                   c->d_phi, c->d_dirty, c->ncopy);
+    k_conv_queue_all<<<blocks_for(c->nsub, 256), 256, 0, c->stream>>>((uint32_t)inst, c->nsub, c->d_act,
+                                                                       c->d_list[0], c->d_cnt);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int fim_step(fim_ctx *c, double phi_start, double cur_time, double dt, double front_width,
+             int32_t *n_active) {
+  return guarded([&] {
+    if (!c) throw Error(FIM_EINVAL, "null context");
+    if (c->o.mode != FIM_MODE_COMPAT) throw Error(FIM_ESTATE, "fim_step needs compat mode");
+    HIPCHECK(hipSetDevice(c->dev));
+    HIPCHECK(hipEventRecord(c->ev0, c->stream));
+    if (c->ngrp)
+      k_compat_sync<<<blocks_for(c->ngrp, 256), 256, 0, c->stream>>>(c->d_grp, c->ngrp, c->d_g2c_dsp,
+                                                                      c->d_g2c, c->d_phi64, c->d_st);
+    k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_pending, 1, 0u);
+    CompatArgs a = c->xa;
+    a.fw = front_width;
+    a.cur = cur_time;
+    a.phi_start = phi_start;
+    a.dt = dt;
+    launch_compat_compute(c, a);
+    HIPCHECK(hipEventRecord(c->ev1, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->h_pin, c->d_pending, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stats.last_ms = ms;
+    if (n_active) *n_active = (int32_t)c->h_pin[0];
+  });
+}
+
+int fim_run(fim_ctx *c, int32_t max_iters, int32_t *iters) {
+  return guarded([&] {
+    if (!c) throw Error(FIM_EINVAL, "null context");
+    if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "fim_run needs converge mode");
+    if (max_iters <= 0) throw Error(FIM_EINVAL, "max_iters must be positive");
+    HIPCHECK(hipSetDevice(c->dev));
+    if (!c->gexec) build_graph(c);
+    HIPCHECK(hipMemsetAsync(c->d_stats, 0, 8 * sizeof(unsigned long long), c->stream));
+    HIPCHECK(hipEventRecord(c->ev0, c->stream));
+    int32_t done = 0;
+    uint32_t left = 1;
+    while (done < max_iters) {
+      HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+      done += c->graph_iters;
+      HIPCHECK(hipMemcpyAsync(c->h_pin, c->d_cnt, 4, hipMemcpyDeviceToHost, c->stream));
+      HIPCHECK(hipStreamSynchronize(c->stream));
+      left = c->h_pin[0];
+      if (left == 0) break;
+    }
+    HIPCHECK(hipEventRecord(c->ev1, c->stream));
+    unsigned long long st[8];
+    HIPCHECK(hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stats.last_ms = ms;
+    c->stats.activations = (int64_t)st[0];
+    c->stats.local_iters = (int64_t)st[1];
+    c->stats.elem_updates = (int64_t)st[2];
+    c->stats.global_iters = (int64_t)st[3];
+    if (iters) *iters = (int32_t)st[3];
+    if (left != 0) throw Error(FIM_ENOCONV, "iteration cap reached before convergence");
+  });
+}
+
+int fim_download(fim_ctx *c, int32_t inst, double *phi, int32_t *state) {
+  return guarded([&] {
+    if (!c) throw Error(FIM_EINVAL, "null context");
+    check_inst(c, inst);
+    HIPCHECK(hipSetDevice(c->dev));
+    const int g = blocks_for(c->nn, 256);
+    if (c->o.mode == FIM_MODE_CONVERGE) {
+      k_conv_gather<<<g, 256, 0, c->stream>>>(c->d_phi, c->d_g2c_dsp, c->d_g2c, c->nn,
+                                              (size_t)inst * c->ncopy, c->d_tmpf);
+      HIPCHECK(hipGetLastError());
+      std::vector<float> h(c->nn);
+      HIPCHECK(hipMemcpyAsync(h.data(), c->d_tmpf, c->nn * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      HIPCHECK(hipStreamSynchronize(c->stream));
+      for (uint32_t v = 0; v < c->nn; ++v) {
+        const bool reached = h[v] < F_INF;
+        if (phi) phi[v] = reached ? (double)h[v] : DBL_MAX;
+        if (state) state[v] = reached ? FIM_FROZEN : FIM_FAR;
+      }
+    } else {
+      k_compat_gather<<<g, 256, 0, c->stream>>>(c->d_phi64, c->d_st, c->d_g2c_dsp, c->d_g2c, c->nn,
+                                                c->d_tmpd, c->d_tmpi);
+      HIPCHECK(hipGetLastError());
+      std::vector<double> hp(c->nn);
+      std::vector<int32_t> hs(c->nn);
+      HIPCHECK(hipMemcpyAsync(hp.data(), c->d_tmpd, c->nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHECK(hipMemcpyAsync(hs.data(), c->d_tmpi, c->nn * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+      HIPCHECK(hipStreamSynchronize(c->stream));
+      if (phi) std::memcpy(phi, hp.data(), c->nn * sizeof(double));
+      if (state) std::memcpy(state, hs.data(), c->nn * sizeof(int32_t));
+    }
+  });
+}
+
+int fim_residual(fim_ctx *c, int32_t inst, double *res) {
+  return guarded([&] {
+    if (!c || !res) throw Error(FIM_EINVAL, "null argument");
+    if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "residual needs converge mode");
+    check_inst(c, inst);
+    HIPCHECK(hipSetDevice(c->dev));
+    k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_tmpu, 1, 0u);
+    k_conv_residual<<<(int)std::min<uint32_t>(c->nsub, 4096u), 256, 0, c->stream>>>(
+        c->d_sub, c->nsub, c->d_e2n, c->d_geo32, c->d_ereg, c->d_rscale, c->d_phi,
+        (size_t)inst * c->ncopy, (uint32_t)inst, c->d_tmpu);
+    HIPCHECK(hipGetLastError());
+    uint32_t bits = 0;
+    HIPCHECK(hipMemcpyAsync(&bits, c->d_tmpu, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    float r;
+    std::memcpy(&r, &bits, 4);
+    *res = r;
+  });
+}
+
+int fim_stats(const fim_ctx *c, fim_stats_t *s) {
+  return guarded([&] {
+    if (!c || !s) throw Error(FIM_EINVAL, "null argument");
+    *s = c->stats;
+  });
+}
+
+void fim_destroy(fim_ctx *c) {
+  if (c) (void)hipSetDevice(c->dev);
+  destroy_ctx(c);
+}
+
+int fim_device_info(int32_t device, char *buf, int64_t len) {
+  return guarded([&] {
+    if (!buf || len <= 0) throw Error(FIM_EINVAL, "null buffer");
+    int n = 0;
+    HIPCHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw Error(FIM_EINVAL, "no such device");
+    hipDeviceProp_t p;
+    HIPCHECK(hipGetDeviceProperties(&p, device));
+    std::snprintf(buf, (size_t)len,
+                  "device %d: %s (%s), %d CUs, %.1f GiB, LDS/block %zu B, wavefront %d, clock %d kHz, "
+                  "devices visible %d",
+                  device, p.name, p.gcnArchName, p.multiProcessorCount,
+                  (double)p.totalGlobalMem / (1024.0 * 1024.0 * 1024.0), (size_t)p.sharedMemPerBlock,
+                  p.warpSize, p.clockRate, n);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,285 @@
+"""fimpy -- ctypes binding of libcufim.so (include/cufim.h).
+
+The solver runs only through the HIP library: loading fails loudly when
+``cufim-nvlink_amd/lib/libcufim.so`` has not been built, and no call falls
+back to CPU code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libcufim.so")
+CLI_PATH = os.path.join(PKG, "bin", "cufim")
+
+FAR, NEIG, NARROW, NEXT, FROZEN, TAIL = range(6)
+MODE_CONVERGE, MODE_COMPAT = 0, 1
+LOCAL_EXACT, LOCAL_REF = 0, 1
+PART_TAGS, PART_RCB = 0, 1
+DMAX = np.finfo(np.float64).max
+
+ERRORS = {-1: "FIM_EINVAL", -2: "FIM_EIO", -3: "FIM_ELIMIT", -4: "FIM_EDEVICE",
+          -5: "FIM_ENOCONV", -6: "FIM_ESTATE"}
+
+_pd = C.POINTER(C.c_double)
+_pf = C.POINTER(C.c_float)
+_pi64 = C.POINTER(C.c_int64)
+_pi32 = C.POINTER(C.c_int32)
+_pu8 = C.POINTER(C.c_uint8)
+_vp = C.c_void_p
+
+
+class FimError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class PartOpts(C.Structure):
+    _fields_ = [("source", C.c_int32), ("max_nn", C.c_int32), ("max_ne", C.c_int32),
+                ("seed", C.c_uint64)]
+
+
+class PackInfo(C.Structure):
+    _fields_ = [("nn_global", C.c_int64), ("ne_global", C.c_int64), ("nsub", C.c_int64),
+                ("ncopy", C.c_int64), ("n_iface_copies", C.c_int64), ("nn_max", C.c_int32),
+                ("ne_max", C.c_int32), ("device_bytes_mesh", C.c_int64), ("lds_bytes", C.c_int64)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("local", C.c_int32), ("n_instances", C.c_int32),
+                ("device", C.c_int32), ("block", C.c_int32), ("local_iters", C.c_int32),
+                ("graph_iters", C.c_int32), ("grid", C.c_int32), ("eps_rel", C.c_double),
+                ("apd", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("global_iters", C.c_int64), ("activations", C.c_int64),
+                ("local_iters", C.c_int64), ("elem_updates", C.c_int64), ("last_ms", C.c_double)]
+
+
+# every symbol include/cufim.h declares, with its ctypes signature
+SIGNATURES = {
+    "fim_last_error": (C.c_char_p, []),
+    "fim_mesh_create": (C.c_int, [C.c_int64, C.c_int64, _pd, _pi64, _pd, _pd, _pi32, C.POINTER(_vp)]),
+    "fim_mesh_load_vtk": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "fim_mesh_read_tags": (C.c_int, [_vp, C.c_char_p]),
+    "fim_mesh_set_tags": (C.c_int, [_vp, _pi32]),
+    "fim_mesh_set_regions": (C.c_int, [_vp, _pu8]),
+    "fim_mesh_info": (C.c_int, [_vp, _pi64, _pi64]),
+    "fim_mesh_get": (C.c_int, [_vp, _pd, _pi64, _pi32]),
+    "fim_mesh_get_material": (C.c_int, [_vp, _pd, _pd]),
+    "fim_mesh_front_width": (C.c_double, [_vp]),
+    "fim_mesh_save_vtk": (C.c_int, [_vp, C.c_char_p, _pd, _pi32]),
+    "fim_mesh_destroy": (None, [_vp]),
+    "fim_mesh_gen_box": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                   C.c_double, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.POINTER(_vp)]),
+    "fim_pack_build": (C.c_int, [_vp, C.POINTER(PartOpts), C.POINTER(_vp)]),
+    "fim_pack_info": (C.c_int, [_vp, C.POINTER(PackInfo)]),
+    "fim_pack_partition": (C.c_int, [_vp, _pi64, _pi64]),
+    "fim_pack_destroy": (None, [_vp]),
+    "fim_create": (C.c_int, [_vp, C.POINTER(Opts), C.POINTER(_vp)]),
+    "fim_reset": (C.c_int, [_vp]),
+    "fim_seed": (C.c_int, [_vp, C.c_int32, _pi64, C.c_int64, C.c_double]),
+    "fim_set_scaling": (C.c_int, [_vp, C.c_int32, _pf]),
+    "fim_init_field": (C.c_int, [_vp, C.c_int32, _pf]),
+    "fim_step": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _pi32]),
+    "fim_run": (C.c_int, [_vp, C.c_int32, _pi32]),
+    "fim_download": (C.c_int, [_vp, C.c_int32, _pd, _pi32]),
+    "fim_residual": (C.c_int, [_vp, C.c_int32, _pd]),
+    "fim_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
+    "fim_destroy": (None, [_vp]),
+    "fim_device_info": (C.c_int, [C.c_int32, C.c_char_p, C.c_int64]),
+}
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built: run `python cufim-nvlink_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise FimError(rc, lib().fim_last_error().decode())
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else C.cast(None, t)
+
+
+class Mesh:
+    def __init__(self, handle):
+        self.h = handle
+        nn, ne = C.c_int64(), C.c_int64()
+        _check(lib().fim_mesh_info(self.h, C.byref(nn), C.byref(ne)))
+        self.nn, self.ne = nn.value, ne.value
+
+    @classmethod
+    def from_arrays(cls, xyz, e2n, fib=None, vel=None, etags=None):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        e2n = np.ascontiguousarray(e2n, dtype=np.int64).reshape(-1, 4)
+        fib = None if fib is None else np.ascontiguousarray(fib, dtype=np.float64)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64)
+        etags = None if etags is None else np.ascontiguousarray(etags, dtype=np.int32)
+        h = _vp()
+        _check(lib().fim_mesh_create(xyz.shape[0], e2n.shape[0], _p(xyz, _pd), _p(e2n, _pi64),
+                                     _p(fib, _pd), _p(vel, _pd), _p(etags, _pi32), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_vtk(cls, path):
+        h = _vp()
+        _check(lib().fim_mesh_load_vtk(path.encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def gen_box(cls, lx, ly, lz, nx, ny, nz, jitter=0.1, seed=0, angle0=0.0, angle1=0.0,
+                vf=0.6, vc=0.6):
+        h = _vp()
+        _check(lib().fim_mesh_gen_box(lx, ly, lz, nx, ny, nz, jitter, seed, angle0, angle1, vf, vc,
+                                      C.byref(h)))
+        return cls(h)
+
+    def read_tags(self, path):
+        _check(lib().fim_mesh_read_tags(self.h, path.encode()))
+
+    def set_tags(self, tags):
+        t = np.ascontiguousarray(tags, dtype=np.int32)
+        assert t.shape[0] == self.ne
+        _check(lib().fim_mesh_set_tags(self.h, _p(t, _pi32)))
+
+    def set_regions(self, reg):
+        r = np.ascontiguousarray(reg, dtype=np.uint8)
+        assert r.shape[0] == self.ne
+        _check(lib().fim_mesh_set_regions(self.h, _p(r, _pu8)))
+
+    def arrays(self):
+        xyz = np.zeros((self.nn, 3))
+        e2n = np.zeros((self.ne, 4), dtype=np.int64)
+        tags = np.zeros(self.ne, dtype=np.int32)
+        _check(lib().fim_mesh_get(self.h, _p(xyz, _pd), _p(e2n, _pi64), _p(tags, _pi32)))
+        return xyz, e2n, tags
+
+    def material(self):
+        fib = np.zeros((self.ne, 3))
+        vel = np.zeros((self.ne, 2))
+        _check(lib().fim_mesh_get_material(self.h, _p(fib, _pd), _p(vel, _pd)))
+        return fib, vel
+
+    def front_width(self):
+        return lib().fim_mesh_front_width(self.h)
+
+    def save_vtk(self, path, phi=None, state=None):
+        p = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+        s = None if state is None else np.ascontiguousarray(state, dtype=np.int32)
+        _check(lib().fim_mesh_save_vtk(self.h, path.encode(), _p(p, _pd), _p(s, _pi32)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.fim_mesh_destroy(self.h)
+            self.h = None
+
+
+class Pack:
+    def __init__(self, mesh: Mesh, source=PART_RCB, max_nn=512, max_ne=0, seed=0):
+        o = PartOpts(source, max_nn, max_ne, seed)
+        self.h = _vp()
+        _check(lib().fim_pack_build(mesh.h, C.byref(o), C.byref(self.h)))
+        i = PackInfo()
+        _check(lib().fim_pack_info(self.h, C.byref(i)))
+        self.info = {k: getattr(i, k) for k, _ in PackInfo._fields_}
+
+    def partition(self):
+        e2s = np.zeros(self.info["ne_global"], dtype=np.int64)
+        c2g = np.zeros(self.info["ncopy"], dtype=np.int64)
+        _check(lib().fim_pack_partition(self.h, _p(e2s, _pi64), _p(c2g, _pi64)))
+        return e2s, c2g
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.fim_pack_destroy(self.h)
+            self.h = None
+
+
+class Solver:
+    def __init__(self, pack: Pack, mode=MODE_CONVERGE, local=LOCAL_EXACT, n_instances=1, device=0,
+                 block=256, local_iters=0, graph_iters=0, grid=0, eps_rel=0.0, apd=0.0):
+        self.pack = pack
+        self.nn = pack.info["nn_global"]
+        self.n_instances = n_instances
+        o = Opts(mode, local, n_instances, device, block, local_iters, graph_iters, grid, eps_rel, apd)
+        self.h = _vp()
+        _check(lib().fim_create(pack.h, C.byref(o), C.byref(self.h)))
+
+    def reset(self):
+        _check(lib().fim_reset(self.h))
+
+    def seed(self, vtx, t0=0.0, inst=0):
+        v = np.ascontiguousarray(vtx, dtype=np.int64).reshape(-1)
+        _check(lib().fim_seed(self.h, inst, _p(v, _pi64), v.shape[0], t0))
+
+    def set_scaling(self, scale, inst=0):
+        s = np.ascontiguousarray(scale, dtype=np.float32)
+        assert s.shape == (4,)
+        _check(lib().fim_set_scaling(self.h, inst, _p(s, _pf)))
+
+    def init_field(self, phi0, inst=0):
+        p = np.ascontiguousarray(phi0, dtype=np.float32)
+        assert p.shape[0] == self.nn
+        _check(lib().fim_init_field(self.h, inst, _p(p, _pf)))
+
+    def step(self, phi_start, cur_time, dt, front_width):
+        n = C.c_int32()
+        _check(lib().fim_step(self.h, phi_start, cur_time, dt, front_width, C.byref(n)))
+        return n.value
+
+    def run(self, max_iters=100000):
+        it = C.c_int32()
+        _check(lib().fim_run(self.h, max_iters, C.byref(it)))
+        return it.value
+
+    def download(self, inst=0):
+        phi = np.zeros(self.nn)
+        st = np.zeros(self.nn, dtype=np.int32)
+        _check(lib().fim_download(self.h, inst, _p(phi, _pd), _p(st, _pi32)))
+        return phi, st
+
+    def residual(self, inst=0):
+        r = C.c_double()
+        _check(lib().fim_residual(self.h, inst, C.byref(r)))
+        return r.value
+
+    def stats(self):
+        s = Stats()
+        _check(lib().fim_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.fim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def device_info(device=0):
+    buf = C.create_string_buffer(512)
+    _check(lib().fim_device_info(device, buf, 512))
+    return buf.value.decode()

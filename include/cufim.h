@@ -1,0 +1,214 @@
+/*
+ * cufim.h -- C ABI of the MI355X FIM activation-time solver (libcufim.so).
+ *
+ * Drop-in boundary for the reference activation-time path.  The reference
+ * exposes that path as a C++ class, not an FFI:
+ *   tw::CudaSolverFIM(const ek_data&)            src/eikonal/cuda_fim.h:1465
+ *   double CudaSolverFIM::solve(act, apd, file, reps)
+ *                                                src/eikonal/cuda_fim.h:1468-1470,
+ *                                                src/eikonal/cuda_fim.cu:2386-2503
+ * The entry points below are what that class does, split at the host/device
+ * line so any host language can bind them with plain pointers and sizes.
+ * The C++ mirror of the class (fim::SolverFIM) and the reference-format CLI
+ * are built on top of this header.
+ *
+ * Conventions
+ *   - every int-returning call returns 0 on success and a negative FIM_E*
+ *     code on failure; fim_last_error() gives a message (thread-local).
+ *     Nothing here calls exit() or abort(), and no C++ exception crosses it.
+ *   - units are the reference's: micrometres, micrometres per microsecond
+ *     (= m/s), microseconds.
+ *   - calls on one fim_ctx must be serialised by the caller; distinct
+ *     contexts are independent.  Host objects (fim_mesh, fim_pack) are
+ *     read-only after construction and may be shared across threads.
+ */
+#ifndef CUFIM_H_
+#define CUFIM_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define FIM_OK 0
+#define FIM_EINVAL (-1)   /* bad argument or inconsistent input          */
+#define FIM_EIO (-2)      /* file not found / unreadable / bad format    */
+#define FIM_ELIMIT (-3)   /* a size limit was exceeded (LDS, u16 ids)    */
+#define FIM_EDEVICE (-4)  /* HIP runtime error                           */
+#define FIM_ENOCONV (-5)  /* iteration cap reached before convergence    */
+#define FIM_ESTATE (-6)   /* call not valid in the context's mode/state  */
+
+/* vertex states, numbering of the reference enum ek_state
+ * (src/core/types.h:85) */
+#define FIM_FAR 0
+#define FIM_NEIG 1
+#define FIM_NARROW 2
+#define FIM_NEXT 3
+#define FIM_FROZEN 4
+#define FIM_TAIL 5
+
+/* solver modes */
+#define FIM_MODE_CONVERGE 0 /* plain FIM to the fixed point, FP32, exact local
+                               solver (tet + tri + edge); batched instances  */
+#define FIM_MODE_COMPAT 1   /* the reference time-window solver, FP64, with the
+                               reference local update as written           */
+
+/* local solver variants */
+#define FIM_LOCAL_EXACT 0 /* minimum over the opposite face (tet, tri, edge) */
+#define FIM_LOCAL_REF 1   /* reference update_per_element
+                             (src/eikonal/cuda_fim.h:1362-1411)            */
+
+/* partition sources for fim_pack_build */
+#define FIM_PART_TAGS 0 /* element tags as given (reference .tags input,
+                           reference submesh numbering, types.cpp:306-570)  */
+#define FIM_PART_RCB 1  /* own seeded recursive bisection, LDS-budgeted,
+                           submesh-contiguous renumbering                  */
+
+const char *fim_last_error(void);
+
+/* ========================================================================
+ * Host-side mesh (replaces tw::mesh_t + load_vtk + read_vector_txt +
+ * compute_base_connectivity, src/core/types.h:221-254, src/io/vtk.cpp:11-105,
+ * src/io/io_base.h:82-86, src/core/types.cpp:43-225)
+ * ===================================================================== */
+typedef struct fim_mesh fim_mesh;
+
+/* Build from arrays (copied).  e2n: 4*ne vertex ids (any order; sorted per
+ * element as the reference does).  fib: 3*ne unit fibre per element or NULL
+ * for (1,0,0).  vel: 2*ne (v_fibre, v_cross) per element or NULL for 0.6.
+ * etags: ne element tags or NULL (all 1). */
+int fim_mesh_create(int64_t nn, int64_t ne, const double *xyz, const int64_t *e2n,
+                    const double *fib, const double *vel, const int32_t *etags,
+                    fim_mesh **out);
+/* legacy ASCII VTK tetrahedral mesh (reference load_vtk semantics: fibres
+ * (1,0,0), speed 0.6) */
+int fim_mesh_load_vtk(const char *path, fim_mesh **out);
+/* partition tags, one integer per element (reference .tags file) */
+int fim_mesh_read_tags(fim_mesh *m, const char *path);
+int fim_mesh_set_tags(fim_mesh *m, const int32_t *atags);
+/* per-element region ids 0..3 (regional conduction-velocity scalings) */
+int fim_mesh_set_regions(fim_mesh *m, const uint8_t *region);
+int fim_mesh_info(const fim_mesh *m, int64_t *nn, int64_t *ne);
+/* copies out xyz (3*nn), element nodes sorted per element (4*ne), element
+ * tags (ne); any pointer may be NULL */
+int fim_mesh_get(const fim_mesh *m, double *xyz, int64_t *e2n_sorted, int32_t *etags);
+/* per-element fibre (3*ne) and speeds (2*ne: v_fibre, v_cross); either may
+ * be NULL */
+int fim_mesh_get_material(const fim_mesh *m, double *fib, double *vel);
+/* reference front width 1.01 * max edge length / 0.6
+ * (src/eikonal/cuda_fim.h:1536-1576) */
+double fim_mesh_front_width(const fim_mesh *m);
+/* write the reference VTK output (src/io/vtk.cpp:109-212): points %.3f,
+ * CELL_TYPES before CELLS, sorted element rows, elemTag, and POINT_DATA
+ * phi / states (values >= 1e50 written as -1).  phi/state may be NULL. */
+int fim_mesh_save_vtk(const fim_mesh *m, const char *path, const double *phi,
+                      const int32_t *state);
+void fim_mesh_destroy(fim_mesh *m);
+
+/* Synthetic meshes (own generators, seeded, deterministic).
+ * Box [0,L]^3 split into nx*ny*nz hexes x 6 tets (Kuhn split along the main
+ * diagonal), interior vertices jittered uniformly by +-jitter*spacing per
+ * axis (SplitMix64-seeded xoshiro256**).  Fibre angle rotates linearly in z
+ * from angle0 to angle1 (degrees, in the xy plane); speeds vf, vc. */
+int fim_mesh_gen_box(double lx, double ly, double lz, int nx, int ny, int nz,
+                     double jitter, uint64_t seed, double angle0, double angle1,
+                     double vf, double vc, fim_mesh **out);
+
+/* ========================================================================
+ * Pack: the submesh decomposition in device layout (replaces
+ * extract_submeshes2 + smsh_t/eksv_t/inter3_t blobs, src/core/types.cpp:
+ * 306-570, src/eikonal/cuda_fim.h:752-908, 1196-1239, cuda_fim.cu:1218-1501)
+ * ===================================================================== */
+typedef struct fim_pack fim_pack;
+
+typedef struct {
+  int32_t source;   /* FIM_PART_TAGS or FIM_PART_RCB                      */
+  int32_t max_nn;   /* RCB: max vertex copies per submesh (e.g. 512)       */
+  int32_t max_ne;   /* RCB: max elements per submesh (0 = derived)         */
+  uint64_t seed;    /* RCB: tie-break seed (the partition is a pure
+                       function of mesh and seed)                          */
+} fim_part_opts;
+
+typedef struct {
+  int64_t nn_global, ne_global;
+  int64_t nsub, ncopy, n_iface_copies;
+  int32_t nn_max, ne_max;
+  int64_t device_bytes_mesh;  /* compact FP32 mesh bytes (plain mode)      */
+  int64_t lds_bytes;          /* dynamic LDS per workgroup (plain mode)    */
+} fim_pack_info_t;
+
+int fim_pack_build(const fim_mesh *m, const fim_part_opts *o, fim_pack **out);
+int fim_pack_info(const fim_pack *p, fim_pack_info_t *info);
+/* element -> submesh id (ne) and local->global vertex ids per copy (ncopy) */
+int fim_pack_partition(const fim_pack *p, int64_t *e2s, int64_t *c2g);
+void fim_pack_destroy(fim_pack *p);
+
+/* ========================================================================
+ * Device solver context (replaces CudaSolverFIM's device side)
+ * ===================================================================== */
+typedef struct fim_ctx fim_ctx;
+
+typedef struct {
+  int32_t mode;          /* FIM_MODE_*                                       */
+  int32_t local;         /* FIM_LOCAL_* (converge mode; compat uses REF
+                            unless set to EXACT)                            */
+  int32_t n_instances;   /* independent solves on the pack (converge mode)   */
+  int32_t device;        /* HIP device ordinal                               */
+  int32_t block;         /* workgroup size: 64, 128, 256 (0 = 256)           */
+  int32_t local_iters;   /* max local iterations per activation (0 = until
+                            locally converged)                              */
+  int32_t graph_iters;   /* global iterations per captured graph (0 = 8)     */
+  int32_t grid;          /* workgroups of the persistent kernels (0 = auto)  */
+  double eps_rel;        /* convergence threshold (0 = 1e-6, reference)      */
+  double apd;            /* compat: action potential duration, us (0 = 3e5)  */
+} fim_opts;
+
+typedef struct {
+  int64_t global_iters;     /* global iterations run by the last fim_run    */
+  int64_t activations;      /* submesh activations (work items)             */
+  int64_t local_iters;      /* local iterations summed over activations     */
+  int64_t elem_updates;     /* element evaluations                          */
+  double last_ms;           /* device time of the last fim_run / fim_step   */
+} fim_stats_t;
+
+int fim_create(const fim_pack *p, const fim_opts *o, fim_ctx **out);
+/* reset_states_gpu (src/eikonal/cuda_fim.cu:479-542): every copy of every
+ * instance to far / +inf */
+int fim_reset(fim_ctx *c);
+/* seed stimulus vertices (global ids) of one instance with time t0 on all
+ * their copies; update_events_gpu (src/eikonal/cuda_fim.cu:632-683) */
+int fim_seed(fim_ctx *c, int32_t inst, const int64_t *vtx, int64_t n, double t0);
+/* converge mode: per-instance, per-region scaling of both speeds (4 regions) */
+int fim_set_scaling(fim_ctx *c, int32_t inst, const float scale[4]);
+/* converge mode: initial upper-bound field (global, nn values) with every
+ * vertex active (warm start) */
+int fim_init_field(fim_ctx *c, int32_t inst, const float *phi0);
+/* compat mode: one reference time window -- sync_step2 then compute_step
+ * (src/eikonal/cuda_fim.cu:2428-2437).  n_active (may be NULL) receives the
+ * number of active vertex copies after the window. */
+int fim_step(fim_ctx *c, double phi_start, double cur_time, double dt,
+             double front_width, int32_t *n_active);
+/* converge mode: run global iterations (interface exchange + per-submesh
+ * local FIM) for all instances until no submesh is active or max_iters is
+ * reached (FIM_ENOCONV).  iters (may be NULL) receives the count. */
+int fim_run(fim_ctx *c, int32_t max_iters, int32_t *iters);
+/* global phi (min over copies; >= 1e50 means unreached, the reference uses
+ * DBL_MAX) and state (max over copies; converge mode reports FROZEN for
+ * reached and FAR for unreached) of one instance; either may be NULL.
+ * Host merge of src/eikonal/cuda_fim.cu:2464-2485, done on the device. */
+int fim_download(fim_ctx *c, int32_t inst, double *phi, int32_t *state);
+/* FIM residual sweep: max over vertices of (phi - update(phi)) / (1 + phi)
+ * for one instance (converge mode), the fixed-point test */
+int fim_residual(fim_ctx *c, int32_t inst, double *res);
+int fim_stats(const fim_ctx *c, fim_stats_t *s);
+void fim_destroy(fim_ctx *c);
+
+/* device description (replaces cuda_info::print, src/core/cuda_base.cu:72-309) */
+int fim_device_info(int32_t device, char *buf, int64_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUFIM_H_ */

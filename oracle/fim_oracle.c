@@ -263,7 +263,7 @@ double orc_local(int variant, const double geo[6], const double ph[4], int t) {
   v = fmin(v, orc_tri_exact(e13, e14, e34, p1, p3));
   v = fmin(v, orc_tri_exact(e23, e24, e34, p2, p3));
   v = fmin(v, orc_tet_exact(e12, e13, e23, e14, e24, e34, p1, p2, p3));
-  return v;
+  return v >= ORC_BIG ? DMAX : v; /* no candidate: the reference sentinel */
 }
 
 /* ------------------------------------------------------------------------ */

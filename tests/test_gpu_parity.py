@@ -1,0 +1,154 @@
+"""GPU parity: the HIP solver (through the C ABI) against the CPU oracle.
+
+Tolerances
+  compat mode (FP64, reference local update): bit-identical to the oracle's
+    restatement on the same partition; against the reference golden
+    (printed with 6 decimals) within 1e-6 us.
+  converge mode (FP32, exact local update): max |dT| <= 1e-4 * max(T)
+    against the FP64 fixed point of the oracle, the per-config bound the
+    solver is specified to.
+"""
+import numpy as np
+import pytest
+
+import cases
+import fimpy as F
+import pyoracle as O
+import vtkio
+
+pytestmark = pytest.mark.gpu
+
+
+def _compat_run(mesh, pack, stims, dt, n_steps, local=F.LOCAL_REF):
+    s = F.Solver(pack, mode=F.MODE_COMPAT, local=local)
+    fw = mesh.front_width()
+    s.reset()
+    for active, tstart, vtx in stims:  # update_events_gpu at cur_time 0
+        if not (tstart >= 0.0 and active):
+            break
+        s.seed(vtx, 0.0)
+    cur = 0.0
+    for _ in range(n_steps):
+        ps = cur
+        cur += dt
+        s.step(ps, cur, dt, fw)
+    phi, st = s.download()
+    s.close()
+    return phi, st
+
+
+def test_compat_ut_mesh_matches_golden_and_oracle():
+    m = F.Mesh.load_vtk(f"{cases.UT}/ut_mesh.vtk")
+    m.read_tags(f"{cases.UT}/ut_mesh.tags")
+    p = F.Pack(m, F.PART_TAGS)
+    stims = cases.ut_plan_stimuli()
+    phi, st = _compat_run(m, p, stims, 150e3, 3)
+    g = vtkio.read_vtk(f"{cases.UT}/cuda_fim_res.vtk")
+    gp, gs = g["point_data"]["phi"], g["point_data"]["states"]
+    assert np.array_equal(st, gs.astype(np.int32))
+    ours = np.where(phi < 1e50, phi, -1.0)
+    assert np.max(np.abs(ours - gp)) <= 1e-6
+    xyz, e2n, _ = m.arrays()
+    om = O.Mesh(xyz, e2n, atags=np.loadtxt(f"{cases.UT}/ut_mesh.tags", dtype=np.int32))
+    ophi, ost, _ = O.compat_solve(om, stims, 150e3, 3)
+    assert np.array_equal(st, ost)
+    assert np.array_equal(phi, ophi)  # bit-identical FP64
+
+
+@pytest.mark.parametrize("steps", [1, 2, 4])
+def test_compat_ut_mesh_windows(steps):
+    m = F.Mesh.load_vtk(f"{cases.UT}/ut_mesh.vtk")
+    m.read_tags(f"{cases.UT}/ut_mesh_p.tags")
+    p = F.Pack(m, F.PART_TAGS)
+    stims = cases.ut_plan_stimuli()
+    phi, st = _compat_run(m, p, stims, 150e3, steps)
+    xyz, e2n, _ = m.arrays()
+    om = O.Mesh(xyz, e2n, atags=np.loadtxt(f"{cases.UT}/ut_mesh_p.tags", dtype=np.int32))
+    ophi, ost, _ = O.compat_solve(om, stims, 150e3, steps)
+    assert np.array_equal(st, ost)
+    assert np.array_equal(phi, ophi)
+
+
+def test_compat_box_rcb_partition():
+    """reference semantics on a generated mesh with the own partitioner"""
+    cfg = cases.scaled(cases.C1, 2)
+    m, om = cases.box(cfg, seed=3)
+    p = F.Pack(m, F.PART_RCB, max_nn=128, seed=3)
+    e2s, _ = p.partition()
+    om = O.Mesh(om.xyz, om.e2n, fib=om.fib, vel=om.vel, atags=e2s.astype(np.int32))
+    stims = [(1, 0.0, [0])]
+    fw = m.front_width()
+    n_steps = int(np.ceil(1.8 * 10000 / 0.6 / 1000.0)) + 2
+    phi, st = _compat_run(m, p, stims, 1000.0, n_steps)
+    ophi, ost, _ = O.compat_solve(om, stims, 1000.0, n_steps, front_width=fw)
+    assert np.array_equal(st, ost)
+    assert np.array_equal(phi, ophi)
+
+
+def _converge(m, pack, sources, **kw):
+    s = F.Solver(pack, **kw)
+    s.seed(sources, 0.0)
+    it = s.run()
+    phi, st = s.download()
+    res = s.residual()
+    stats = s.stats()
+    s.close()
+    return phi, st, it, res, stats
+
+
+def _check_fp32(phi, ref):
+    fin = ref < 1e50
+    assert np.array_equal(fin, phi < 1e50)
+    err = np.max(np.abs(phi[fin] - ref[fin]))
+    tol = 1e-4 * ref[fin].max()
+    assert err <= tol, (err, tol)
+    return err
+
+
+def test_converge_config1_vs_fp64_oracle():
+    m, om = cases.box(cases.C1, seed=1)
+    p = F.Pack(m, F.PART_RCB, max_nn=512, seed=1)
+    phi, st, it, res, stats = _converge(m, p, [0])
+    ref, _ = O.fim_solve(om, [0], variant=O.LOCAL_EXACT)
+    _check_fp32(phi, ref)
+    assert res < 1e-5
+    assert np.all(st == F.FROZEN)
+
+
+def test_converge_anisotropic_slab_vs_fp64_oracle():
+    cfg = cases.scaled(cases.C2, 4)
+    m, om = cases.box(cfg, seed=2)
+    p = F.Pack(m, F.PART_RCB, max_nn=512, seed=2)
+    src = cases.pacing_sites(cfg, 5, seed=2)
+    phi, st, it, res, stats = _converge(m, p, src)
+    ref, _ = O.fim_solve(om, src, variant=O.LOCAL_EXACT)
+    _check_fp32(phi, ref)
+
+
+@pytest.mark.parametrize("block", [64, 128, 256, 512])
+def test_converge_workgroup_shapes_agree(block):
+    cfg = cases.scaled(cases.C2, 8)
+    m, om = cases.box(cfg, seed=5)
+    p = F.Pack(m, F.PART_RCB, max_nn=256, seed=5)
+    src = cases.pacing_sites(cfg, 3, seed=5)
+    phi, *_ = _converge(m, p, src, block=block)
+    ref, _ = O.fim_solve(om, src, variant=O.LOCAL_EXACT)
+    _check_fp32(phi, ref)
+
+
+def test_converge_deterministic_and_batched():
+    cfg = cases.scaled(cases.C2, 8)
+    m, om = cases.box(cfg, seed=7)
+    p = F.Pack(m, F.PART_RCB, max_nn=256, seed=7)
+    srcs = [cases.pacing_sites(cfg, k + 1, seed=100 + k) for k in range(4)]
+    singles = [_converge(m, p, s)[0] for s in srcs]
+    again = _converge(m, p, srcs[0])[0]
+    assert np.array_equal(singles[0], again)  # run-to-run bitwise
+    b = F.Solver(p, n_instances=4)
+    for i, s in enumerate(srcs):
+        b.seed(s, 0.0, inst=i)
+    b.run()
+    for i in range(4):
+        phi, _ = b.download(inst=i)
+        assert np.array_equal(phi, singles[i])  # batching changes nothing
+    b.close()

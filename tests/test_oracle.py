@@ -1,0 +1,376 @@
+"""Pins the CPU oracle (oracle/fim_oracle.c) against the reference's golden
+output, the reference's own host code (oracle/_ref, when built), closed forms
+and brute force.  No GPU needed."""
+import numpy as np
+import pytest
+
+import cases
+import pyoracle as O
+import pyref as R
+import vtkio
+
+UTV = f"{cases.UT}/ut_mesh.vtk"
+UTT = f"{cases.UT}/ut_mesh.tags"
+need_ref = pytest.mark.skipif(not R.available(), reason="oracle/_ref not built (needs reference sources)")
+
+
+def _ut():
+    g = vtkio.read_vtk(UTV)
+    return g["xyz"], g["cells"], np.loadtxt(UTT, dtype=np.int32)
+
+
+# ---------------------------------------------------------------- golden ----
+def test_golden_replay_reference_variant():
+    """data/output/cuda_fim_res.vtk = ut_mesh, ut_mesh.tags, plan stimuli,
+    dt 150 ms, 3 windows (src/eikonal/cuda_fim.cu:3077-3130 settings)."""
+    xyz, cells, tags = _ut()
+    m = O.Mesh(xyz, cells, atags=tags)
+    phi, st, stats = O.compat_solve(m, cases.ut_plan_stimuli(), 150e3, 3)
+    g = vtkio.read_vtk(f"{cases.UT}/cuda_fim_res.vtk")
+    assert np.array_equal(st, g["point_data"]["states"].astype(np.int32))
+    ours = np.where(phi < 1e50, phi, -1.0)
+    # the golden prints %lf (6 decimals)
+    assert np.max(np.abs(ours - g["point_data"]["phi"])) <= 5.0001e-7
+    assert np.bincount(st, minlength=6)[[O.FAR, O.NEXT, O.FROZEN, O.TAIL]].tolist() == [60, 21, 113, 38]
+
+
+def test_golden_rejects_exact_tet_variant():
+    """the golden was produced with the reference's tet formula: the exact
+    tet candidate gives measurably different times, so the replay above
+    really pins the reference variant"""
+    xyz, cells, tags = _ut()
+    m = O.Mesh(xyz, cells, atags=tags)
+    phi, st, _ = O.compat_solve(m, cases.ut_plan_stimuli(), 150e3, 3, variant=O.LOCAL_EXACT)
+    g = vtkio.read_vtk(f"{cases.UT}/cuda_fim_res.vtk")
+    ours = np.where(phi < 1e50, phi, -1.0)
+    assert np.max(np.abs(ours - g["point_data"]["phi"])) > 1.0
+
+
+def test_golden_planted_mistakes_fail():
+    """plausible mistakes in the restated window logic change the replay"""
+    xyz, cells, tags = _ut()
+    m = O.Mesh(xyz, cells, atags=tags)
+    g = vtkio.read_vtk(f"{cases.UT}/cuda_fim_res.vtk")
+    gs = g["point_data"]["states"].astype(np.int32)
+    # one window fewer / more
+    for n in (2, 4):
+        _, st, _ = O.compat_solve(m, cases.ut_plan_stimuli(), 150e3, n)
+        assert not np.array_equal(st, gs)
+    # APD off by 10 %
+    _, st, _ = O.compat_solve(m, cases.ut_plan_stimuli(), 150e3, 3, apd=3.3e5)
+    assert not np.array_equal(st, gs)
+    # upwind filter without the front-width allowance (phi > phi_start):
+    # the front stalls at window starts.  (The golden does not pin the
+    # width's exact value: any width >= 50 ms replays it.)
+    phi, st, _ = O.compat_solve(m, cases.ut_plan_stimuli(), 150e3, 3, front_width=0.0)
+    assert not np.array_equal(st, gs)
+
+
+# ------------------------------------------------- reference host code ----
+@need_ref
+def test_connectivity_matches_reference_host_code():
+    for xyz, cells in (_ut()[:2], _box_arrays()):
+        R.set_mesh(xyz, cells)
+        re2n, rc = R.connectivity(xyz.shape[0], cells.shape[0])
+        oe2n, oc = O.connectivity(cells)
+        assert np.array_equal(re2n, oe2n)
+        for a, b in zip(rc, oc):
+            assert np.array_equal(a, b)
+
+
+@need_ref
+def test_geometry_bit_exact_against_reference_host_code():
+    xyz, cells = _box_arrays()
+    R.set_mesh(xyz, cells)
+    R.connectivity(xyz.shape[0], cells.shape[0])
+    rg = R.geo(cells.shape[0])
+    oe2n, _ = O.connectivity(cells)
+    M = O.metric([1.0, 0.0, 0.0], 0.6, 0.6)
+    og = np.array([O.geo_elem(xyz, oe2n[e], M) for e in range(cells.shape[0])])
+    assert np.array_equal(rg, og)
+
+
+@need_ref
+def test_submesh_numbering_matches_reference_host_code():
+    for tagfile in ("ut_mesh.tags", "ut_mesh_p.tags"):
+        xyz, cells, _ = _ut()
+        tags = np.loadtxt(f"{cases.UT}/{tagfile}", dtype=np.int32)
+        R.set_mesh(xyz, cells, tags)
+        R.connectivity(xyz.shape[0], cells.shape[0])
+        subs, (cnt, ldx, adx) = R.extract(xyz.shape[0])
+        oe2n, _ = O.connectivity(cells)
+        nagg, e2a, dsp, l2g = O.extract_numbering(oe2n, xyz.shape[0], tags)
+        assert nagg == len(subs)
+        for a in range(nagg):
+            assert np.array_equal(subs[a]["l2g"], l2g[dsp[a]:dsp[a + 1]])
+        # n2i: copies of each vertex in ascending submesh order
+        k = 0
+        for v in range(xyz.shape[0]):
+            mine = [(int(np.where(l2g[dsp[a]:dsp[a + 1]] == v)[0][0]), a)
+                    for a in range(nagg) if v in l2g[dsp[a]:dsp[a + 1]]]
+            theirs = [(int(ldx[k + i]), int(adx[k + i])) for i in range(cnt[v])]
+            assert mine == theirs
+            k += cnt[v]
+
+
+@need_ref
+def test_front_width_matches_reference_formula():
+    xyz, cells, _ = _ut()
+    oe2n, (c0, d0, n2e, nc, nd, n2n) = O.connectivity(cells)
+    # restate compute_dwf over n2n directly
+    best = 0.0
+    for v in range(xyz.shape[0]):
+        nb = n2n[nd[v]:nd[v] + nc[v]]
+        best = max(best, float(np.max(np.sum((xyz[nb] - xyz[v]) ** 2, axis=1))))
+    assert O.front_width(oe2n, xyz) == 1.01 * np.sqrt(best) / 0.6
+
+
+def _box_arrays():
+    rng = np.random.default_rng(4)
+    n = 4
+    g = np.stack(np.meshgrid(*[np.arange(n + 1)] * 3, indexing="ij"), -1).reshape(-1, 3).astype(float)
+    g = g * 700.0 + rng.uniform(-70, 70, g.shape)
+    idx = lambda i, j, k: (i * (n + 1) + j) * (n + 1) + k  # noqa: E731
+    cells = []
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                for p in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+                    c = [i, j, k]
+                    t = [idx(*c)]
+                    for ax in p:
+                        c[ax] += 1
+                        t.append(idx(*c))
+                    cells.append(t[::-1])  # unsorted on purpose
+    return g, np.array(cells, dtype=np.int64)
+
+
+# ------------------------------------------------------ local solvers ----
+# relabel tables of the reference (src/eikonal/cuda_fim.h:41-48), as data
+REF_ADF = [4, 5, 3, 1, 3, 0, 4, 0, 2, 1, 2, 5]
+REF_BCE = [2, 0, 1, 5, 2, 4, 3, 5, 1, 0, 3, 4]
+REF_PHI = [1, 2, 3, 2, 3, 0, 3, 0, 1, 0, 1, 2]
+
+
+def test_relabel_equals_reference_tables():
+    for t in range(4):
+        r = O.relabel(t)
+        assert r[0:3] == REF_ADF[3 * t:3 * t + 3]
+        assert r[3:6] == REF_BCE[3 * t:3 * t + 3]
+        assert r[6:9] == REF_PHI[3 * t:3 * t + 3]
+
+
+def _rand_tet(rng, aniso=True):
+    x = rng.normal(size=(4, 3)) * 1000.0
+    f = rng.normal(size=3)
+    f /= np.linalg.norm(f)
+    vf, vs = (0.6, 0.24) if aniso else (0.6, 0.6)
+    M = np.outer(f, f) / vf ** 2 + (np.eye(3) - np.outer(f, f)) / vs ** 2
+    return x, M, O.metric(f, vf, vs)
+
+
+def test_relabel_gram_identity():
+    """reference test_gray_code (cuda_fim.cu:2595-2681): coefficients from
+    relabelled squared lengths equal direct products e_i^T M e_j"""
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        x, M, Mp = _rand_tet(rng)
+        geo = O.geo_elem(x, np.arange(4), Mp)
+        for t in range(4):
+            r = O.relabel(t)
+            q1, q2, q3 = r[6:9]
+            e13, e23, e34, e12, e14, e24 = (geo[i] for i in r[0:6])
+            A, B, u = x[q1] - x[q3], x[q2] - x[q3], x[t] - x[q3]
+            assert np.isclose(e13, A @ M @ A, rtol=1e-12)
+            assert np.isclose(e23, B @ M @ B, rtol=1e-12)
+            assert np.isclose(e34, u @ M @ u, rtol=1e-12)
+            assert np.isclose(0.5 * (e13 + e23 - e12), A @ M @ B, rtol=1e-9, atol=1e-6)
+            assert np.isclose(0.5 * (e13 + e34 - e14), A @ M @ u, rtol=1e-9, atol=1e-6)
+            assert np.isclose(0.5 * (e23 + e34 - e24), B @ M @ u, rtol=1e-9, atol=1e-6)
+
+
+def _seg_brute(x1, x2, x4, M, p1, p2, n=200001):
+    lam = np.linspace(0.0, 1.0, n)
+    pts = x1[None] + lam[:, None] * (x2 - x1)[None]
+    d = x4[None] - pts
+    val = p1 + lam * (p2 - p1) + np.sqrt(np.einsum("ij,jk,ik->i", d, M, d))
+    i = int(np.argmin(val))
+    return val[i], lam[i]
+
+
+def test_tri_candidates_against_brute_force():
+    rng = np.random.default_rng(1)
+    checked = 0
+    for _ in range(300):
+        x, M, Mp = _rand_tet(rng)
+        g = O.geo_elem(x, np.arange(4), Mp)  # e12 e13 e23 e14 e24 e34
+        e12, e14, e24 = g[0], g[3], g[4]
+        p1 = rng.uniform(0, 2000)
+        p2 = p1 + rng.uniform(-0.9, 0.9) * np.sqrt(e12)
+        bv, bl = _seg_brute(x[0], x[1], x[3], M, p1, p2)
+        ex = O.tri_exact(e12, e14, e24, p1, p2)
+        if 1e-3 < bl < 1 - 1e-3:
+            assert ex < O.BIG
+            assert abs(ex - bv) <= 1e-7 * (1 + bv)
+            # reference triangle (FP32 intermediates) on the same triangle:
+            # its arguments are (|e12|^2, |e14|^2, |e24|^2) in the roles
+            # (c, a+2b+c, a) -> see src/eikonal/cuda_fim.h:1398
+            rf = O.tri_ref(e12, e14, e24, p1, p2)
+            assert abs(rf - bv) <= 3e-6 * (1 + bv)
+            checked += 1
+        elif ex < O.BIG:
+            assert ex >= bv - 1e-7 * (1 + bv)
+    assert checked > 50
+
+
+def _face_brute(x, M, p, t=3, n=600):
+    """min over the closed face opposite x[t] of interpolated p + distance"""
+    q = [i for i in range(4) if i != t]
+    best = (np.inf, None)
+    a = np.linspace(0, 1, n)
+    A, B = np.meshgrid(a, a, indexing="ij")
+    mask = A + B <= 1.0
+    l1, l2 = A[mask], B[mask]
+    l3 = 1 - l1 - l2
+    pts = l1[:, None] * x[q[0]] + l2[:, None] * x[q[1]] + l3[:, None] * x[q[2]]
+    d = x[t][None] - pts
+    val = l1 * p[q[0]] + l2 * p[q[1]] + l3 * p[q[2]] + np.sqrt(np.einsum("ij,jk,ik->i", d, M, d))
+    i = int(np.argmin(val))
+    best = (val[i], (l1[i], l2[i], l3[i]))
+    return best
+
+
+def test_exact_local_update_against_brute_force():
+    """orc_local(EXACT) = minimum over the whole opposite face (tet interior,
+    edges, vertices) -- a 2-D minimisation checked on a dense grid"""
+    rng = np.random.default_rng(2)
+    interior = 0
+    for _ in range(120):
+        x, M, Mp = _rand_tet(rng)
+        geo = O.geo_elem(x, np.arange(4), Mp)
+        p = np.zeros(4)
+        # upwind-ish times: a plane wave plus noise
+        nrm = rng.normal(size=3)
+        p[:3] = (x[:3] @ nrm) / np.linalg.norm(nrm) / 0.5 + rng.uniform(0, 300, 3)
+        p[:3] -= p[:3].min()
+        t = 3
+        bv, bl = _face_brute(x, M, p, t)
+        v = O.local(O.LOCAL_EXACT, geo, p, t)
+        # grid spacing 1/600 of the face: objective is Lipschitz, so the grid
+        # minimum is within a small margin above the true minimum
+        assert v <= bv + 1e-9 * (1 + bv)
+        assert v >= bv - 2e-3 * (1 + bv)
+        if min(bl) > 0.02:
+            interior += 1
+            assert abs(v - bv) <= 1e-3 * (1 + bv)
+    assert interior > 10
+
+
+def test_exact_tet_reproduces_plane_waves():
+    """a plane wave T = n.x / c (|n|_{M^-1} = 1 direction) is reproduced
+    exactly when its characteristic crosses the face"""
+    rng = np.random.default_rng(3)
+    hits = 0
+    for _ in range(400):
+        x, M, Mp = _rand_tet(rng)
+        geo = O.geo_elem(x, np.arange(4), Mp)
+        # slowness vector s with s^T M^-1 s = 1 -> T(x) = s.x
+        s = rng.normal(size=3)
+        s /= np.sqrt(s @ np.linalg.inv(M) @ s)
+        p = x @ s
+        p -= p.min()
+        if np.argmax(p) != 3:
+            continue
+        # characteristic direction M^-1 s from x4 backwards must hit the face
+        dirn = np.linalg.inv(M) @ s
+        A = np.stack([x[0] - x[2], x[1] - x[2], -dirn], 1)
+        l1, l2, tau = np.linalg.solve(A, x[3] - x[2])
+        if not (l1 > 0.01 and l2 > 0.01 and l1 + l2 < 0.99 and tau > 0):
+            continue
+        v = O.local(O.LOCAL_EXACT, geo, p, 3)
+        assert abs(v - p[3]) <= 1e-9 * (1 + abs(p[3]))
+        hits += 1
+    assert hits > 10
+
+
+def test_reference_tet_formula_rarely_accepts():
+    """the reference tet candidate (coefficients named inconsistently,
+    src/eikonal/cuda_fim.h:1307-1357) almost never yields a candidate on
+    plane waves where the exact tet does: its tri/edge candidates carry the
+    reference results"""
+    rng = np.random.default_rng(5)
+    acc_ref = acc_exact = 0
+    for _ in range(2000):
+        x, M, Mp = _rand_tet(rng, aniso=False)
+        geo = O.geo_elem(x, np.arange(4), Mp)
+        s = rng.normal(size=3)
+        s /= np.sqrt(s @ np.linalg.inv(M) @ s)
+        p = x @ s
+        p -= p.min()
+        r = O.relabel(3)
+        e13, e23, e34, e12, e14, e24 = (geo[i] for i in r[0:6])
+        p1, p2, p3 = (p[i] for i in r[6:9])
+        acc_ref += O.tet_ref(e13, e23, e34, e12, e14, e24, p1, p2, p3) < 1e300
+        acc_exact += O.tet_exact(e12, e13, e23, e14, e24, e34, p1, p2, p3) < 1e300
+    assert acc_exact > 100
+    assert acc_ref < 0.05 * acc_exact
+
+
+# -------------------------------------------------- fixed-point solver ----
+def test_fixed_point_bounds_and_residual():
+    """FP64 fixed point: zero residual; above the continuous lower bound
+    (metric distance to the source); below the edge-graph shortest path"""
+    cfg = cases.scaled(cases.C1, 2)
+    import fimpy as F  # generator only (host code, no GPU)
+    m = F.Mesh.gen_box(seed=11, **cfg)
+    xyz, e2n, _ = m.arrays()
+    om = O.Mesh(xyz, e2n)
+    phi, _ = O.fim_solve(om, [0])
+    assert O.residual(om, phi) == 0.0
+    lb = np.linalg.norm(xyz - xyz[0], axis=1) / 0.6
+    assert np.all(phi >= lb - 1e-9)
+    # Dijkstra on element edges (edge candidates only)
+    import heapq
+    nb = [set() for _ in range(xyz.shape[0])]
+    for c in e2n:
+        for a in c:
+            nb[a].update(int(b) for b in c if b != a)
+    dist = np.full(xyz.shape[0], np.inf)
+    dist[0] = 0.0
+    h = [(0.0, 0)]
+    while h:
+        d, v = heapq.heappop(h)
+        if d > dist[v]:
+            continue
+        for u in nb[v]:
+            nd = d + np.linalg.norm(xyz[u] - xyz[v]) / 0.6
+            if nd < dist[u]:
+                dist[u] = nd
+                heapq.heappush(h, (nd, u))
+    assert np.all(phi <= dist + 1e-9)
+    assert np.mean(phi < dist - 1.0) > 0.5  # faces and tets do improve on edges
+
+
+def test_fixed_point_is_order_independent():
+    """the label-correcting result equals plain Jacobi sweeps to a fixed
+    point (tiny mesh)"""
+    cfg = dict(cases.C1, nx=4, ny=4, nz=4)
+    import fimpy as F
+    m = F.Mesh.gen_box(seed=12, **cfg)
+    xyz, e2n, _ = m.arrays()
+    om = O.Mesh(xyz, e2n)
+    phi, _ = O.fim_solve(om, [0])
+    M = O.metric([1, 0, 0], 0.6, 0.6)
+    geo = np.array([O.geo_elem(xyz, c, M) for c in e2n])
+    cur = np.full(xyz.shape[0], O.DMAX)
+    cur[0] = 0.0
+    for _ in range(200):
+        new = cur.copy()
+        for e, c in enumerate(e2n):
+            ph = np.where(cur[c] < O.DMAX, cur[c], O.BIG)
+            for t in range(4):
+                new[c[t]] = min(new[c[t]], O.local(O.LOCAL_EXACT, geo[e], ph, t))
+        if np.array_equal(new, cur):
+            break
+        cur = new
+    assert np.max(np.abs(cur - phi)) <= 1e-9 * phi.max()
