@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""bench.py -- activation maps/s of the MI355X FIM solver.
+
+Workload (one GPU): the anisotropic slab -- 160 x 160 x 13 hexes x 6 tets =
+1,996,800 tets, 362,894 vertices over 10 x 10 x 2 cm, fibres rotating
+-60..+60 degrees through the thickness, v_fibre 0.6 and v_cross 0.24 m/s --
+with five pacing vertices per map on the z = 0 face.  One step = one batch of
+B maps (different pacing sets) solved to the eps = 1e-6 fixed point by the
+batched HIP solver, mesh resident in HBM; reset and seeding are inside the
+timed region.  The map of the first instance is checked against the FP64
+fixed point of the CPU oracle (max |dT| <= 0.1 ms is required for a valid
+number), and that oracle run, timed on one host core, is the CPU baseline.
+Plain single-solve GPU FIM (B = 1) is timed beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cufim-nvlink_amd"))
+
+import fimpy as F  # noqa: E402
+from fimpy import workloads as W  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+METRIC = "maps/s @max|ΔT|≤0.1ms, 2M-tet slab: batched+NN warm-start vs plain FIM"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pacing_sets(cfg, batch, seed):
+    return [W.pacing_sites(cfg, 5, seed * 100003 + i) for i in range(batch)]
+
+
+def solve_batch(s, sets):
+    s.reset()
+    for i, src in enumerate(sets):
+        s.seed(src, 0.0, inst=i)
+    return s.run(max_iters=1_000_000)
+
+
+def traffic_from_profiles():
+    """HBM bytes per compute launch from the committed PMC summary."""
+    p = os.path.join(ROOT, "profiles", "conv_compute_slab_pmc.md")
+    if not os.path.exists(p):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read())
+    return float(m.group(1)) if m else None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--max-nn", type=int, default=512)
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one MI355X only: the batched solve would shard by map across "
+              "GPUs with no exchange, but no multi-GPU run is part of this build.", file=sys.stderr)
+        sys.exit(1)
+
+    cfg = W.SLAB
+    t0 = time.perf_counter()
+    mesh = W.box(cfg, args.seed)
+    pack = F.Pack(mesh, F.PART_RCB, max_nn=args.max_nn, seed=args.seed)
+    info = pack.info
+    log(f"[bench] mesh {mesh.nn} vertices {mesh.ne} tets, {info['nsub']} submeshes, "
+        f"setup {time.perf_counter() - t0:.2f}s")
+    sets = pacing_sets(cfg, args.batch, args.seed)
+
+    # ---- batched solver, timed (graph-captured iterations) -------------
+    s = F.Solver(pack, n_instances=args.batch, block=args.block)
+    for i in range(args.warmup):
+        solve_batch(s, sets)
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    tot = dict(elem_updates=0, global_iters=0, activations=0, local_iters=0)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        solve_batch(s, sets)
+        st = s.stats()
+        for k in tot:
+            tot[k] += st[k]
+    elapsed = time.perf_counter() - t0
+    log(f"[bench] timed {args.steps} steps of {args.batch} maps in {elapsed:.3f}s")
+    phi0, _ = s.download(inst=0)
+    res0 = s.residual(inst=0)
+    s.close()
+
+    # ---- the same batch once more, every compute launch timed with HIP
+    # events on its stream (same kernel sequence as the graph) ------------
+    sp = F.Solver(pack, n_instances=args.batch, block=args.block, profile=True)
+    solve_batch(sp, sets)
+    prof = sp.stats()
+    phi0p, _ = sp.download(inst=0)
+    sp.close()
+    assert np.array_equal(phi0, phi0p), "profiled pass differs from the timed pass"
+    log(f"[bench] profiled pass: {prof['compute_launches']} compute launches, "
+        f"{prof['compute_ms']:.3f} ms")
+    ms_per_step = 1e3 * elapsed / args.steps
+    maps_per_s = args.batch * args.steps / elapsed
+
+    # ---- plain single-solve GPU FIM (the denominator of the target) -----
+    p1 = F.Solver(pack, n_instances=1, block=args.block)
+    solve_batch(p1, sets[:1])
+    t0 = time.perf_counter()
+    for i in range(args.plain_steps):
+        solve_batch(p1, sets[i % len(sets):i % len(sets) + 1])
+    plain_s = (time.perf_counter() - t0) / args.plain_steps
+    phi_plain, _ = p1.download(inst=0)
+    p1.close()
+    log(f"[bench] plain single solve {1e3 * plain_s:.2f} ms")
+
+    # ---- accuracy and CPU baseline (FP64 oracle, one host core) ---------
+    max_dt = None
+    cpu = None
+    if not (args.no_check and args.no_cpu_baseline):
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O  # checker / CPU baseline only
+        xyz, e2n, _ = mesh.arrays()
+        fib, vel = mesh.material()
+        om = O.Mesh(xyz, e2n, fib=fib, vel=vel)
+        t0 = time.perf_counter()
+        ref, ost = O.fim_solve(om, sets[0], variant=O.LOCAL_EXACT)
+        cpu_s = time.perf_counter() - t0
+        fin = ref < 1e50
+        if not np.array_equal(fin, phi0 < 1e50):
+            max_dt = float("inf")
+        else:
+            max_dt = float(np.max(np.abs(phi0[fin] - ref[fin])))
+        log(f"[bench] oracle {cpu_s:.2f}s, max|dT| = {max_dt:.3f} us (max T {ref[fin].max():.0f} us)")
+        cpu = {"value": 1.0 / cpu_s, "unit": "maps/s", "cores": 1, "kind": "port",
+               "sample": "one map of the same slab and pacing set (instance 0), FP64 "
+                         "label-correcting fixed point of the same local update, one host core"}
+    valid = max_dt is not None and max_dt <= 100.0
+
+    bytes_per_launch = prof["alg_bytes"] / max(prof["compute_launches"], 1)
+    avg_launch_s = prof["compute_ms"] / 1e3 / max(prof["compute_launches"], 1)
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": maps_per_s if (valid or args.no_check) else 0.0,
+        "unit": "maps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "anisotropic slab 160x160x13 hexes x 6 = 1,996,800 tets / 362,894 vertices, "
+                        "fibres -60..+60 deg, v_f 0.6 / v_c 0.24 m/s, 5 pacing sites per map; "
+                        f"batch of {args.batch} maps per step solved to eps 1e-6 "
+                        "(batched FIM; NN warm start not included)",
+            "maps_per_step": args.batch,
+            "partition": f"recursive bisection, <= {args.max_nn} vertex copies per submesh, "
+                         f"{info['nsub']} submeshes",
+            "workgroup": args.block,
+        },
+        "max_abs_dT_us": max_dt,
+        "tolerance_us": 100.0,
+        "valid": valid,
+        "residual_inst0": res0,
+        "plain_maps_per_s": 1.0 / plain_s,
+        "batched_over_plain": maps_per_s * plain_s,
+        "solves_per_s": maps_per_s,
+        "elem_evals_per_s": tot["elem_updates"] / elapsed,
+        "global_iters_per_step": tot["global_iters"] / args.steps,
+        "activations_per_step": tot["activations"] / args.steps,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_conv_compute",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(),
+            "alg_bytes_per_launch": bytes_per_launch,
+            "avg_launch_us": 1e6 * avg_launch_s,
+            "launches": prof["compute_launches"],
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -75,7 +75,8 @@ struct ConvArgs {
   uint32_t *zero_count;  // counter of the list after next: zeroed here.  It
                          // was consumed by the previous iteration and is
                          // appended to only from the next iteration on.
-  unsigned long long *stats;  // activations, sweeps, element evaluations, iterations
+  unsigned long long *stats;  // activations, sweeps, element evaluations,
+                              // iterations, algorithmic bytes
   uint32_t nsub, ncopy;
   float eps;
   int32_t max_local;
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
     *a.zero_count = 0;
     if (n_work > 0) atomicAdd(&a.stats[3], 1ull);
   }
-  unsigned long long sweeps = 0, evals = 0, acts = 0;
+  unsigned long long sweeps = 0, evals = 0, acts = 0, abytes = 0;
 
   for (uint32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
     const uint32_t entry = a.list[w];
@@ -241,6 +242,7 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
     }
     sweeps += it;
     acts += 1;
+    abytes += 32ull * ne + 10ull * nn + 12ull * si.ni;
     // write back decreased copies; remember them for the interface push
     for (uint32_t l = tid; l < nn; l += BS) {
       if (flag_at(tch, l)) {
@@ -261,6 +263,7 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
     atomicAdd(&a.stats[0], acts);
     atomicAdd(&a.stats[1], sweeps);
     atomicAdd(&a.stats[2], evals);
+    atomicAdd(&a.stats[4], abytes);
   }
 }
 
@@ -686,6 +689,7 @@ struct fim_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> kev;  // profile: events around each captured compute launch
   ConvArgs ca{};
   SyncArgs sa{};
   CompatArgs xa{};
@@ -740,8 +744,9 @@ int blocks_for(size_t n, int bs) { return (int)std::max<size_t>(1, std::min<size
 
 // one global iteration k: work lists rotate over three buffers; iteration k
 // reads list k%3, appends to (k+1)%3 and zeroes the counter of (k+2)%3
-void enqueue_iteration(fim_ctx *c, int k) {
+void enqueue_iteration(fim_ctx *c, int k, bool timed) {
   const int p = k % 3, q = (k + 1) % 3, z = (k + 2) % 3;
+  if (timed) HIPCHECK(hipEventRecord(c->kev[2 * k], c->stream));
   ConvArgs a = c->ca;
   a.list = c->d_list[p];
   a.count = c->d_cnt + p;
@@ -749,6 +754,7 @@ void enqueue_iteration(fim_ctx *c, int k) {
   a.next_count = c->d_cnt + q;
   a.zero_count = c->d_cnt + z;
   launch_conv_compute(c, a);
+  if (timed) HIPCHECK(hipEventRecord(c->kev[2 * k + 1], c->stream));
   SyncArgs s = c->sa;
   s.list = c->d_list[p];
   s.count = c->d_cnt + p;
@@ -761,7 +767,7 @@ void enqueue_iteration(fim_ctx *c, int k) {
 void build_graph(fim_ctx *c) {
   HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   try {
-    for (int i = 0; i < c->graph_iters; ++i) enqueue_iteration(c, i);
+    for (int i = 0; i < c->graph_iters; ++i) enqueue_iteration(c, i, false);
   } catch (...) {
     hipGraph_t g;
     (void)hipStreamEndCapture(c->stream, &g);
@@ -790,6 +796,8 @@ void destroy_ctx(fim_ctx *c) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->kev)
+    if (e) (void)hipEventDestroy(e);
   for (void *p : c->owned) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1071,16 +1079,36 @@ int fim_run(fim_ctx *c, int32_t max_iters, int32_t *iters) {
     if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "fim_run needs converge mode");
     if (max_iters <= 0) throw Error(FIM_EINVAL, "max_iters must be positive");
     HIPCHECK(hipSetDevice(c->dev));
-    if (!c->gexec) build_graph(c);
+    // profile mode launches the same sequence of kernels without the graph
+    // (events cannot be timed inside a graph) and times every compute launch
+    if (c->o.profile && c->kev.empty()) {
+      c->kev.assign(2 * c->graph_iters, nullptr);
+      for (auto &e : c->kev) HIPCHECK(hipEventCreate(&e));
+    }
+    if (!c->o.profile && !c->gexec) build_graph(c);
     HIPCHECK(hipMemsetAsync(c->d_stats, 0, 8 * sizeof(unsigned long long), c->stream));
     HIPCHECK(hipEventRecord(c->ev0, c->stream));
     int32_t done = 0;
     uint32_t left = 1;
+    double kms = 0.0;
+    int64_t klaunch = 0;
     while (done < max_iters) {
-      HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+      if (c->o.profile) {
+        for (int i = 0; i < c->graph_iters; ++i) enqueue_iteration(c, i, true);
+      } else {
+        HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+      }
       done += c->graph_iters;
       HIPCHECK(hipMemcpyAsync(c->h_pin, c->d_cnt, 4, hipMemcpyDeviceToHost, c->stream));
       HIPCHECK(hipStreamSynchronize(c->stream));
+      if (c->o.profile) {
+        for (int j = 0; j < c->graph_iters; ++j) {
+          float ms = 0;
+          HIPCHECK(hipEventElapsedTime(&ms, c->kev[2 * j], c->kev[2 * j + 1]));
+          kms += ms;
+          ++klaunch;
+        }
+      }
       left = c->h_pin[0];
       if (left == 0) break;
     }
@@ -1095,6 +1123,9 @@ int fim_run(fim_ctx *c, int32_t max_iters, int32_t *iters) {
     c->stats.local_iters = (int64_t)st[1];
     c->stats.elem_updates = (int64_t)st[2];
     c->stats.global_iters = (int64_t)st[3];
+    c->stats.alg_bytes = (int64_t)st[4];
+    c->stats.compute_ms = kms;
+    c->stats.compute_launches = klaunch;
     if (iters) *iters = (int32_t)st[3];
     if (left != 0) throw Error(FIM_ENOCONV, "iteration cap reached before convergence");
   });

@@ -1,0 +1,64 @@
+"""Synthetic workloads (sizes in micrometres, speeds in micrometres per
+microsecond = m/s, times in microseconds), seeded and deterministic."""
+from __future__ import annotations
+
+import numpy as np
+
+from . import Mesh
+
+# unit cube 1 cm, 20^3 hexes x 6 tets (48,000 tets, 9,261 vertices), interior
+# vertices jittered +-10 % of the spacing, isotropic 0.6 m/s, pacing at the
+# origin
+CUBE = dict(lx=10000.0, ly=10000.0, lz=10000.0, nx=20, ny=20, nz=20, jitter=0.1,
+            angle0=0.0, angle1=0.0, vf=0.6, vc=0.6)
+# slab 10 x 10 x 2 cm, 160 x 160 x 13 hexes x 6 tets (1,996,800 tets,
+# 362,894 vertices), fibres rotating -60..+60 degrees through the thickness,
+# v_fibre 0.6, v_cross 0.24 m/s, five pacing vertices on z = 0
+SLAB = dict(lx=100000.0, ly=100000.0, lz=20000.0, nx=160, ny=160, nz=13, jitter=0.1,
+            angle0=-60.0, angle1=60.0, vf=0.6, vc=0.24)
+
+
+def splitmix64(seed):
+    """Deterministic stream of 64-bit integers (SplitMix64)."""
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    while True:
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        yield z ^ (z >> 31)
+
+
+def pick(n_choices, k, seed):
+    """k distinct indices in [0, n_choices) drawn from SplitMix64(seed)."""
+    g = splitmix64(seed)
+    out = []
+    while len(out) < k:
+        v = next(g) % n_choices
+        if v not in out:
+            out.append(v)
+    return out
+
+
+def scaled(cfg, f):
+    """Same physical box with the hex counts divided by f."""
+    c = dict(cfg)
+    for k in ("nx", "ny", "nz"):
+        c[k] = max(2, int(round(cfg[k] / f)))
+    return c
+
+
+def box(cfg, seed) -> Mesh:
+    return Mesh.gen_box(seed=seed, **cfg)
+
+
+def face_vertices(cfg, z_index=0):
+    NX, NY = cfg["nx"] + 1, cfg["ny"] + 1
+    return np.array([i + NX * (j + NY * z_index) for j in range(NY) for i in range(NX)],
+                    dtype=np.int64)
+
+
+def pacing_sites(cfg, k, seed):
+    """k distinct vertices of the z = 0 face, from the seed."""
+    face = face_vertices(cfg, 0)
+    return face[pick(face.shape[0], k, seed)]

@@ -163,6 +163,9 @@ typedef struct {
   int32_t grid;          /* workgroups of the persistent kernels (0 = auto)  */
   double eps_rel;        /* convergence threshold (0 = 1e-6, reference)      */
   double apd;            /* compat: action potential duration, us (0 = 3e5)  */
+  int32_t profile;       /* converge: time every compute-kernel launch with
+                            HIP events recorded inside the graph            */
+  int32_t reserved;
 } fim_opts;
 
 typedef struct {
@@ -171,6 +174,11 @@ typedef struct {
   int64_t local_iters;      /* local iterations summed over activations     */
   int64_t elem_updates;     /* element evaluations                          */
   double last_ms;           /* device time of the last fim_run / fim_step   */
+  int64_t alg_bytes;        /* algorithmic HBM bytes of the activations:
+                               32 per element + 10 per vertex copy + 12 per
+                               interface copy of each activated submesh     */
+  int64_t compute_launches; /* compute-kernel launches timed (profile)      */
+  double compute_ms;        /* summed compute-kernel time (profile)         */
 } fim_stats_t;
 
 int fim_create(const fim_pack *p, const fim_opts *o, fim_ctx **out);

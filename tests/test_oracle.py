@@ -211,15 +211,15 @@ def test_tri_candidates_against_brute_force():
         ex = O.tri_exact(e12, e14, e24, p1, p2)
         if 1e-3 < bl < 1 - 1e-3:
             assert ex < O.BIG
-            assert abs(ex - bv) <= 1e-7 * (1 + bv)
+            assert abs(ex - bv) <= 1e-7 * (1 + abs(bv))
             # reference triangle (FP32 intermediates) on the same triangle:
             # its arguments are (|e12|^2, |e14|^2, |e24|^2) in the roles
             # (c, a+2b+c, a) -> see src/eikonal/cuda_fim.h:1398
             rf = O.tri_ref(e12, e14, e24, p1, p2)
-            assert abs(rf - bv) <= 3e-6 * (1 + bv)
+            assert abs(rf - bv) <= 3e-6 * (1 + abs(bv))
             checked += 1
         elif ex < O.BIG:
-            assert ex >= bv - 1e-7 * (1 + bv)
+            assert ex >= bv - 1e-7 * (1 + abs(bv))
     assert checked > 50
 
 
@@ -271,7 +271,7 @@ def test_exact_tet_reproduces_plane_waves():
     exactly when its characteristic crosses the face"""
     rng = np.random.default_rng(3)
     hits = 0
-    for _ in range(400):
+    for _ in range(3000):
         x, M, Mp = _rand_tet(rng)
         geo = O.geo_elem(x, np.arange(4), Mp)
         # slowness vector s with s^T M^-1 s = 1 -> T(x) = s.x
@@ -283,14 +283,14 @@ def test_exact_tet_reproduces_plane_waves():
             continue
         # characteristic direction M^-1 s from x4 backwards must hit the face
         dirn = np.linalg.inv(M) @ s
-        A = np.stack([x[0] - x[2], x[1] - x[2], -dirn], 1)
+        A = np.stack([x[0] - x[2], x[1] - x[2], dirn], 1)
         l1, l2, tau = np.linalg.solve(A, x[3] - x[2])
         if not (l1 > 0.01 and l2 > 0.01 and l1 + l2 < 0.99 and tau > 0):
             continue
         v = O.local(O.LOCAL_EXACT, geo, p, 3)
         assert abs(v - p[3]) <= 1e-9 * (1 + abs(p[3]))
         hits += 1
-    assert hits > 10
+    assert hits > 30
 
 
 def test_reference_tet_formula_rarely_accepts():
