@@ -64,11 +64,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--max-nn", type=int, default=512)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--max-nn", type=int, default=2048)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--graph-steps", type=int, default=2)
+    ap.add_argument("--staged", action="store_true",
+                    help="stage whole submeshes in LDS (default: element data from L2)")
+    ap.add_argument("--window", type=float, default=2000.0,
+                    help="time window per global iteration in us (0 = plain FIM order)")
+    # plain single-solve GPU FIM: its own best setting (fewer, smaller work
+    # items want smaller submeshes and a finer window)
+    ap.add_argument("--plain-max-nn", type=int, default=256)
+    ap.add_argument("--plain-window", type=float, default=1000.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
@@ -88,16 +97,30 @@ def main():
         f"setup {time.perf_counter() - t0:.2f}s")
     sets = pacing_sets(cfg, args.batch, args.seed)
 
-    # ---- batched solver, timed (graph-captured iterations) -------------
-    s = F.Solver(pack, n_instances=args.batch, block=args.block)
+    # Every solve below launches its iterations individually with a HIP
+    # event pair around each compute-kernel launch on the solver's stream
+    # (profile mode: the same kernel sequence the graph replays).  The
+    # roofline uses the average over all those launches, which is also what
+    # `rocprofv3 --kernel-trace --stats` reports for this command.
+    kern = dict(alg_bytes=0, compute_ms=0.0, compute_launches=0)
+
+    def account(st):
+        for k in kern:
+            kern[k] += st[k]
+
+    # ---- batched solver, timed -----------------------------------------
+    s = F.Solver(pack, n_instances=args.batch, block=args.block, profile=True,
+                 window=args.window, staged=args.staged)
     for i in range(args.warmup):
         solve_batch(s, sets)
+        account(s.stats())
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
-    tot = dict(elem_updates=0, global_iters=0, activations=0, local_iters=0)
+    tot = dict(elem_updates=0, global_iters=0, activations=0, local_iters=0, vertex_updates=0)
     t0 = time.perf_counter()
     for i in range(args.steps):
         solve_batch(s, sets)
         st = s.stats()
+        account(st)
         for k in tot:
             tot[k] += st[k]
     elapsed = time.perf_counter() - t0
@@ -105,28 +128,33 @@ def main():
     phi0, _ = s.download(inst=0)
     res0 = s.residual(inst=0)
     s.close()
-
-    # ---- the same batch once more, every compute launch timed with HIP
-    # events on its stream (same kernel sequence as the graph) ------------
-    sp = F.Solver(pack, n_instances=args.batch, block=args.block, profile=True)
-    solve_batch(sp, sets)
-    prof = sp.stats()
-    phi0p, _ = sp.download(inst=0)
-    sp.close()
-    assert np.array_equal(phi0, phi0p), "profiled pass differs from the timed pass"
-    log(f"[bench] profiled pass: {prof['compute_launches']} compute launches, "
-        f"{prof['compute_ms']:.3f} ms")
     ms_per_step = 1e3 * elapsed / args.steps
     maps_per_s = args.batch * args.steps / elapsed
 
+    # ---- the same batch with graph-captured iterations (library default)
+    g = F.Solver(pack, n_instances=args.batch, block=args.block, window=args.window,
+                 staged=args.staged)
+    solve_batch(g, sets)
+    t0 = time.perf_counter()
+    for i in range(args.graph_steps):
+        solve_batch(g, sets)
+    graph_maps_per_s = args.batch * args.graph_steps / (time.perf_counter() - t0)
+    phi0g, _ = g.download(inst=0)
+    g.close()
+    assert np.array_equal(phi0, phi0g), "graph replay differs from direct launches"
+
     # ---- plain single-solve GPU FIM (the denominator of the target) -----
-    p1 = F.Solver(pack, n_instances=1, block=args.block)
+    ppack = pack if args.plain_max_nn == args.max_nn else \
+        F.Pack(mesh, F.PART_RCB, max_nn=args.plain_max_nn, seed=args.seed)
+    p1 = F.Solver(ppack, n_instances=1, block=args.block, profile=True, window=args.plain_window,
+                  staged=args.staged)
     solve_batch(p1, sets[:1])
+    account(p1.stats())
     t0 = time.perf_counter()
     for i in range(args.plain_steps):
         solve_batch(p1, sets[i % len(sets):i % len(sets) + 1])
+        account(p1.stats())
     plain_s = (time.perf_counter() - t0) / args.plain_steps
-    phi_plain, _ = p1.download(inst=0)
     p1.close()
     log(f"[bench] plain single solve {1e3 * plain_s:.2f} ms")
 
@@ -153,8 +181,8 @@ def main():
                          "label-correcting fixed point of the same local update, one host core"}
     valid = max_dt is not None and max_dt <= 100.0
 
-    bytes_per_launch = prof["alg_bytes"] / max(prof["compute_launches"], 1)
-    avg_launch_s = prof["compute_ms"] / 1e3 / max(prof["compute_launches"], 1)
+    bytes_per_launch = kern["alg_bytes"] / max(kern["compute_launches"], 1)
+    avg_launch_s = kern["compute_ms"] / 1e3 / max(kern["compute_launches"], 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     out = {
         "metric": METRIC,
@@ -178,15 +206,20 @@ def main():
             "partition": f"recursive bisection, <= {args.max_nn} vertex copies per submesh, "
                          f"{info['nsub']} submeshes",
             "workgroup": args.block,
+            "window_us": args.window,
+            "staged": args.staged,
         },
         "max_abs_dT_us": max_dt,
         "tolerance_us": 100.0,
         "valid": valid,
         "residual_inst0": res0,
+        "graph_maps_per_s": graph_maps_per_s,
         "plain_maps_per_s": 1.0 / plain_s,
+        "plain_config": {"max_nn": args.plain_max_nn, "window_us": args.plain_window},
         "batched_over_plain": maps_per_s * plain_s,
         "solves_per_s": maps_per_s,
         "elem_evals_per_s": tot["elem_updates"] / elapsed,
+        "vertex_updates_per_s": tot["vertex_updates"] / elapsed,
         "global_iters_per_step": tot["global_iters"] / args.steps,
         "activations_per_step": tot["activations"] / args.steps,
         "roofline": {
@@ -199,7 +232,7 @@ def main():
             "traffic": traffic_from_profiles(),
             "alg_bytes_per_launch": bytes_per_launch,
             "avg_launch_us": 1e6 * avg_launch_s,
-            "launches": prof["compute_launches"],
+            "launches": kern["compute_launches"],
         },
         "cpu_baseline": cpu,
     }

@@ -68,6 +68,7 @@ struct ConvArgs {
   uint8_t *dirty;        // n_inst x ncopy
   uint8_t *chg;          // n_inst x ncopy
   uint32_t *act;         // n_inst x nsub
+  float *wake;           // n_inst x nsub: lower bound of the dirty vertex times
   const uint32_t *list;  // current work list
   const uint32_t *count;
   uint32_t *next_list;   // next work list (requeue on the sweep cap)
@@ -77,23 +78,37 @@ struct ConvArgs {
                          // appended to only from the next iteration on.
   unsigned long long *stats;  // activations, sweeps, element evaluations,
                               // iterations, algorithmic bytes
+  const uint32_t *iter;  // global iteration counter (advanced by the sync kernel)
+  const uint32_t *n2e_dsp;  // per copy: incident local elements (unstaged kernel)
+  const uint16_t *n2e_le;
   uint32_t nsub, ncopy;
   float eps;
+  float window;          // per-iteration time window, 0 = off
   int32_t max_local;
   // LDS carve (bytes)
-  uint32_t o_phin, o_flg, o_tch, o_e2n, o_geo, o_list, o_misc;
+  uint32_t o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc;
 };
 
-template <int BS>
+// One workgroup per work item (instance, submesh).  STAGED copies the
+// submesh's elements (nodes + geometry) into LDS and finds the active
+// elements by scanning them every sweep; otherwise element data stays in
+// global memory (L2 / MALL resident) and the active elements are marked
+// from the flagged vertices through the vertex -> element adjacency, so an
+// activation costs in proportion to the band of the submesh it touches.
+template <int BS, bool STAGED>
 __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   float *phi_l = reinterpret_cast<float *>(lds);
   uint32_t *phi_n = reinterpret_cast<uint32_t *>(lds + a.o_phin);
   uint32_t *flg = reinterpret_cast<uint32_t *>(lds + a.o_flg);
   uint32_t *tch = reinterpret_cast<uint32_t *>(lds + a.o_tch);
-  uint2 *el = reinterpret_cast<uint2 *>(lds + a.o_e2n);
-  float *gl = reinterpret_cast<float *>(lds + a.o_geo);
+  uint32_t *dfr = reinterpret_cast<uint32_t *>(lds + a.o_dfr);
+  uint2 *el = reinterpret_cast<uint2 *>(lds + a.o_e2n);      // STAGED
+  float *gl = reinterpret_cast<float *>(lds + a.o_geo);       // STAGED
+  uint32_t *emark = reinterpret_cast<uint32_t *>(lds + a.o_e2n);  // !STAGED
+  uint32_t *eseen = reinterpret_cast<uint32_t *>(lds + a.o_geo);  // !STAGED
   uint16_t *elist = reinterpret_cast<uint16_t *>(lds + a.o_list);
+  uint16_t *vlist = reinterpret_cast<uint16_t *>(lds + a.o_vlist);  // !STAGED
   uint32_t *misc = reinterpret_cast<uint32_t *>(lds + a.o_misc);
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
@@ -102,7 +117,9 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
     *a.zero_count = 0;
     if (n_work > 0) atomicAdd(&a.stats[3], 1ull);
   }
-  unsigned long long sweeps = 0, evals = 0, acts = 0, abytes = 0;
+  unsigned long long sweeps = 0, evals = 0, acts = 0, abytes = 0, vupd = 0;
+  // time window of this iteration: flagged vertices later than W wait
+  const float W = a.window > 0.0f ? a.window * (float)(*a.iter + 1u) : F_INF;
 
   for (uint32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
     const uint32_t entry = a.list[w];
@@ -111,66 +128,133 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
     const size_t cb = (size_t)inst * a.ncopy + si.coff;
     const uint32_t nn = si.nn, ne = si.ne;
     const uint32_t nv64 = (nn + 63u) & ~63u;
+    const uint32_t new32 = (ne + 31u) >> 5;
     if (tid == 0) a.act[entry] = 0u;
+    // nothing in this submesh can act before the window reaches its wake
+    // time: keep it queued without staging anything
+    if (a.wake[entry] > W) {
+      if (tid == 0 && atomicExch(&a.act[entry], 1u) == 0u) {
+        const uint32_t pos = atomicAdd(a.next_count, 1u);
+        a.next_list[pos] = entry;
+      }
+      continue;
+    }
     // stage vertex times and dirty flags
     for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
       const uint32_t l = l0 + tid;
-      bool d = false;
+      bool d = false, late = false;
       if (l < nn) {
         const float v = a.phi[cb + l];
         phi_l[l] = v;
         phi_n[l] = __float_as_uint(v);
         d = a.dirty[cb + l] != 0;
         if (d) a.dirty[cb + l] = 0;
+        late = d && v > W;
+        d = d && !late;
       }
-      const uint64_t m = __ballot(d);
+      const uint64_t m = __ballot(d), md = __ballot(late);
       const uint32_t wb = l0 + wave * 64;
       if (wb < nv64) {
         if (lane == 0) flg[wb >> 5] = (uint32_t)m;
         if (lane == 1) flg[(wb >> 5) + 1] = (uint32_t)(m >> 32);
         if (lane == 2) tch[wb >> 5] = 0u;
         if (lane == 3) tch[(wb >> 5) + 1] = 0u;
+        if (lane == 4) dfr[wb >> 5] = (uint32_t)md;
+        if (lane == 5) dfr[(wb >> 5) + 1] = (uint32_t)(md >> 32);
       }
     }
-    // stage elements (geometry scaled by the instance's regional factor)
     const float *rs = a.rscale + 4 * (size_t)inst;
-    for (uint32_t e = tid; e < ne; e += BS) {
-      const size_t ge = (size_t)si.eoff + e;
-      el[e] = a.e2n[ge];
-      const float sc = rs[a.ereg[ge] & 3];
-      const float2 *g2 = reinterpret_cast<const float2 *>(a.geo + 6 * ge);
-      const float2 x0 = g2[0], x1 = g2[1], x2 = g2[2];
-      float *o = gl + 6 * e;
-      o[0] = x0.x * sc;
-      o[1] = x0.y * sc;
-      o[2] = x1.x * sc;
-      o[3] = x1.y * sc;
-      o[4] = x2.x * sc;
-      o[5] = x2.y * sc;
+    if (STAGED) {
+      // stage elements (geometry scaled by the instance's regional factor)
+      for (uint32_t e = tid; e < ne; e += BS) {
+        const size_t ge = (size_t)si.eoff + e;
+        el[e] = a.e2n[ge];
+        const float sc = rs[a.ereg[ge] & 3];
+        const float2 *g2 = reinterpret_cast<const float2 *>(a.geo + 6 * ge);
+        const float2 x0 = g2[0], x1 = g2[1], x2 = g2[2];
+        float *o = gl + 6 * e;
+        o[0] = x0.x * sc;
+        o[1] = x0.y * sc;
+        o[2] = x1.x * sc;
+        o[3] = x1.y * sc;
+        o[4] = x2.x * sc;
+        o[5] = x2.y * sc;
+      }
+    } else {
+      for (uint32_t k = tid; k < new32; k += BS) {
+        emark[k] = 0u;
+        eseen[k] = 0u;
+      }
     }
     __syncthreads();
 
     int it = 0;
     bool capped = false;
     for (;;) {
-      if (tid == 0) misc[0] = 0u;
-      __syncthreads();
-      // active elements: any node flagged (wave ballot + per-wave base)
-      for (uint32_t e0 = 0; e0 < ne; e0 += BS) {
-        const uint32_t e = e0 + tid;
-        bool on = false;
-        if (e < ne) {
-          const uint2 q = el[e];
-          on = flag_at(flg, q.x & 0xffffu) | flag_at(flg, q.x >> 16) |
-               flag_at(flg, q.y & 0xffffu) | flag_at(flg, q.y >> 16);
-        }
-        const uint64_t m = __ballot(on);
-        uint32_t base = 0;
-        if (lane == 0 && m) base = atomicAdd(&misc[0], (uint32_t)__popcll(m));
-        base = __shfl(base, 0);
-        if (on) elist[base + __popcll(m & lanes_below())] = (uint16_t)e;
+      if (tid == 0) {
+        misc[0] = 0u;
+        misc[2] = 0u;
       }
       __syncthreads();
+      if (STAGED) {
+        // active elements: any node flagged (wave ballot + per-wave base)
+        for (uint32_t e0 = 0; e0 < ne; e0 += BS) {
+          const uint32_t e = e0 + tid;
+          bool on = false;
+          if (e < ne) {
+            const uint2 q = el[e];
+            on = flag_at(flg, q.x & 0xffffu) | flag_at(flg, q.x >> 16) |
+                 flag_at(flg, q.y & 0xffffu) | flag_at(flg, q.y >> 16);
+          }
+          const uint64_t m = __ballot(on);
+          uint32_t base = 0;
+          if (lane == 0 && m) base = atomicAdd(&misc[0], (uint32_t)__popcll(m));
+          base = __shfl(base, 0);
+          if (on) elist[base + __popcll(m & lanes_below())] = (uint16_t)e;
+        }
+        __syncthreads();
+      } else {
+        // flagged vertices -> compact list
+        for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
+          const uint32_t l = l0 + tid;
+          const bool on = l < nn && flag_at(flg, l);
+          const uint64_t m = __ballot(on);
+          uint32_t base = 0;
+          if (lane == 0 && m) base = atomicAdd(&misc[2], (uint32_t)__popcll(m));
+          base = __shfl(base, 0);
+          if (on) vlist[base + __popcll(m & lanes_below())] = (uint16_t)l;
+        }
+        __syncthreads();
+        const uint32_t nsrc = misc[2];
+        // mark the elements incident to them
+        for (uint32_t i = tid; i < nsrc; i += BS) {
+          const uint32_t c = si.coff + vlist[i];
+          const uint32_t kb = a.n2e_dsp[c], ke = a.n2e_dsp[c + 1];
+          for (uint32_t k = kb; k < ke; ++k) {
+            const uint32_t le = a.n2e_le[k];
+            atomicOr(&emark[le >> 5], 1u << (le & 31));
+          }
+        }
+        __syncthreads();
+        // marked elements -> compact list (and clear the marks)
+        for (uint32_t k0 = 0; k0 < new32; k0 += BS) {
+          const uint32_t k = k0 + tid;
+          uint32_t bits = 0;
+          if (k < new32) {
+            bits = emark[k];
+            emark[k] = 0u;
+            eseen[k] |= bits;
+          }
+          const uint32_t cnt = __popc(bits);
+          uint32_t base = cnt ? atomicAdd(&misc[0], cnt) : 0u;
+          while (bits) {
+            const uint32_t b = __ffs(bits) - 1;
+            bits &= bits - 1;
+            elist[base++] = (uint16_t)(k * 32 + b);
+          }
+        }
+        __syncthreads();
+      }
       const uint32_t nact = misc[0];
       if (nact == 0) break;
       if ((a.max_local > 0 && it >= a.max_local) || it >= kHardLocalCap) {
@@ -182,11 +266,26 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
       // candidates of the active elements
       for (uint32_t i = tid; i < nact; i += BS) {
         const uint32_t e = elist[i];
-        const uint2 q = el[e];
-        const uint32_t nd[4] = {q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16};
+        uint2 q;
         float g[6];
+        if (STAGED) {
+          q = el[e];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) g[k] = gl[6 * e + k];
+          for (int k = 0; k < 6; ++k) g[k] = gl[6 * e + k];
+        } else {
+          const size_t ge = (size_t)si.eoff + e;
+          q = a.e2n[ge];
+          const float sc = rs[a.ereg[ge] & 3];
+          const float2 *g2 = reinterpret_cast<const float2 *>(a.geo + 6 * ge);
+          const float2 x0 = g2[0], x1 = g2[1], x2 = g2[2];
+          g[0] = x0.x * sc;
+          g[1] = x0.y * sc;
+          g[2] = x1.x * sc;
+          g[3] = x1.y * sc;
+          g[4] = x2.x * sc;
+          g[5] = x2.y * sc;
+        }
+        const uint32_t nd[4] = {q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16};
         float ph[4];
         int f[4];
 #pragma unroll
@@ -199,18 +298,22 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
         // if it is later than the earliest other node
         if (nf - f[0] > 0 && ph[0] > fminf(ph[1], fminf(ph[2], ph[3]))) {
           const float c = cand_exact<0, float>(g, ph, F_INF);
+          ++vupd;
           if (c < ph[0]) atomicMin(&phi_n[nd[0]], __float_as_uint(c));
         }
         if (nf - f[1] > 0 && ph[1] > fminf(ph[0], fminf(ph[2], ph[3]))) {
           const float c = cand_exact<1, float>(g, ph, F_INF);
+          ++vupd;
           if (c < ph[1]) atomicMin(&phi_n[nd[1]], __float_as_uint(c));
         }
         if (nf - f[2] > 0 && ph[2] > fminf(ph[0], fminf(ph[1], ph[3]))) {
           const float c = cand_exact<2, float>(g, ph, F_INF);
+          ++vupd;
           if (c < ph[2]) atomicMin(&phi_n[nd[2]], __float_as_uint(c));
         }
         if (nf - f[3] > 0 && ph[3] > fminf(ph[0], fminf(ph[1], ph[2]))) {
           const float c = cand_exact<3, float>(g, ph, F_INF);
+          ++vupd;
           if (c < ph[3]) atomicMin(&phi_n[nd[3]], __float_as_uint(c));
         }
       }
@@ -219,7 +322,7 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
       // |old-new| >= eps (1 + (old+new)/2), cuda_fim.cu:2154)
       for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
         const uint32_t l = l0 + tid;
-        bool sig = false, dec = false;
+        bool sig = false, dec = false, late = false;
         if (l < nn) {
           const float ov = phi_l[l];
           const float nv = __uint_as_float(phi_n[l]);
@@ -227,32 +330,61 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
             phi_l[l] = nv;
             dec = true;
             sig = (ov - nv) >= a.eps * (1.0f + 0.5f * (ov + nv));
+            late = sig && nv > W;
+            sig = sig && !late;
           }
         }
-        const uint64_t m = __ballot(sig), mt = __ballot(dec);
+        const uint64_t m = __ballot(sig), mt = __ballot(dec), md = __ballot(late);
         const uint32_t wb = l0 + wave * 64;
         if (wb < nv64) {
           if (lane == 0) flg[wb >> 5] = (uint32_t)m;
           if (lane == 1) flg[(wb >> 5) + 1] = (uint32_t)(m >> 32);
           if (lane == 2) tch[wb >> 5] |= (uint32_t)mt;
           if (lane == 3) tch[(wb >> 5) + 1] |= (uint32_t)(mt >> 32);
+          // a deferred vertex that is now a source leaves the deferred set
+          if (lane == 4) dfr[wb >> 5] = (dfr[wb >> 5] & ~(uint32_t)m) | (uint32_t)md;
+          if (lane == 5) dfr[(wb >> 5) + 1] = (dfr[(wb >> 5) + 1] & ~(uint32_t)(m >> 32)) | (uint32_t)(md >> 32);
         }
       }
       __syncthreads();
     }
     sweeps += it;
     acts += 1;
-    abytes += 32ull * ne + 10ull * nn + 12ull * si.ni;
-    // write back decreased copies; remember them for the interface push
+    // algorithmic bytes of the activation: copies read and written (phi 4+4,
+    // flag 1+1), interface entries (index 4, read 4, write 4) and the
+    // element data used (nodes 8 + geometry 24) -- all elements when staged,
+    // the distinct elements evaluated otherwise
+    uint32_t used = ne;
+    if (!STAGED) {
+      if (tid == 0) misc[3] = 0u;
+      __syncthreads();
+      uint32_t c = 0;
+      for (uint32_t k = tid; k < new32; k += BS) c += __popc(eseen[k]);
+      if (c) atomicAdd(&misc[3], c);
+      __syncthreads();
+      used = misc[3];
+    }
+    abytes += 32ull * used + 10ull * nn + 12ull * si.ni;
+    // write back decreased copies; remember them for the interface push.
+    // Deferred vertices (and sources left by the sweep cap) stay dirty and
+    // the submesh is queued for the next iteration.
+    if (tid == 0) misc[1] = __float_as_uint(F_INF);
+    __syncthreads();
     for (uint32_t l = tid; l < nn; l += BS) {
       if (flag_at(tch, l)) {
         a.phi[cb + l] = phi_l[l];
         a.chg[cb + l] = 1;
       }
-      if (capped && flag_at(flg, l)) a.dirty[cb + l] = 1;
+      if (flag_at(dfr, l) || (capped && flag_at(flg, l))) {
+        a.dirty[cb + l] = 1;
+        atomicMin(&misc[1], __float_as_uint(phi_l[l]));
+      }
     }
-    if (capped && tid == 0) {
-      if (atomicExch(&a.act[entry], 1u) == 0u) {
+    __syncthreads();
+    if (tid == 0) {
+      const float wk = __uint_as_float(misc[1]);
+      a.wake[entry] = wk;
+      if (wk < F_INF && atomicExch(&a.act[entry], 1u) == 0u) {
         const uint32_t pos = atomicAdd(a.next_count, 1u);
         a.next_list[pos] = entry;
       }
@@ -265,6 +397,8 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
     atomicAdd(&a.stats[2], evals);
     atomicAdd(&a.stats[4], abytes);
   }
+  for (int off = 32; off > 0; off >>= 1) vupd += __shfl_xor(vupd, off);
+  if (lane == 0 && vupd) atomicAdd(&a.stats[5], vupd);
 }
 
 struct SyncArgs {
@@ -277,16 +411,20 @@ struct SyncArgs {
   uint8_t *dirty;
   uint8_t *chg;
   uint32_t *act;
+  float *wake;
   const uint32_t *list;
   const uint32_t *count;
   uint32_t *next_list;
   uint32_t *next_count;
+  uint32_t *iter;
   uint32_t nsub, ncopy;
 };
 
 // interface push: min over copies, order independent
 __global__ __launch_bounds__(256) void k_conv_sync(SyncArgs a) {
   const uint32_t n_work = *a.count;
+  // the compute kernel of this iteration has finished reading the counter
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.iter += 1u;
   for (uint32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
     const uint32_t entry = a.list[w];
     const uint32_t inst = entry / a.nsub, s = entry - inst * a.nsub;
@@ -306,6 +444,7 @@ __global__ __launch_bounds__(256) void k_conv_sync(SyncArgs a) {
           if (v < old) {
             a.dirty[ib + oc] = 1;
             const uint32_t e2 = inst * a.nsub + a.c2s[oc];
+            atomicMin(reinterpret_cast<uint32_t *>(a.wake + e2), v);
             if (atomicExch(&a.act[e2], 1u) == 0u) {
               const uint32_t pos = atomicAdd(a.next_count, 1u);
               a.next_list[pos] = e2;
@@ -321,13 +460,15 @@ __global__ __launch_bounds__(256) void k_conv_sync(SyncArgs a) {
 // their submesh is queued on the current list
 __global__ void k_conv_seed(const uint32_t *copies, uint32_t n, uint32_t inst, float t0,
                             const uint32_t *c2s, float *phi, uint8_t *dirty, uint32_t *act,
-                            uint32_t *list, uint32_t *count, uint32_t nsub, uint32_t ncopy) {
+                            float *wake, uint32_t *list, uint32_t *count, uint32_t nsub,
+                            uint32_t ncopy) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t c = copies[i];
     const size_t gc = (size_t)inst * ncopy + c;
     atomicMin(reinterpret_cast<uint32_t *>(phi + gc), __float_as_uint(t0));
     dirty[gc] = 1;
     const uint32_t e = inst * nsub + c2s[c];
+    atomicMin(reinterpret_cast<uint32_t *>(wake + e), __float_as_uint(t0));
     if (atomicExch(&act[e], 1u) == 0u) list[atomicAdd(count, 1u)] = e;
   }
 }
@@ -343,10 +484,11 @@ __global__ void k_conv_init(const float *phi0, const uint32_t *c2g, uint32_t ins
   }
 }
 
-__global__ void k_conv_queue_all(uint32_t inst, uint32_t nsub, uint32_t *act, uint32_t *list,
-                                 uint32_t *count) {
+__global__ void k_conv_queue_all(uint32_t inst, uint32_t nsub, uint32_t *act, float *wake,
+                                 uint32_t *list, uint32_t *count) {
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nsub; s += gridDim.x * blockDim.x) {
     const uint32_t e = inst * nsub + s;
+    wake[e] = 0.0f;
     if (atomicExch(&act[e], 1u) == 0u) list[atomicAdd(count, 1u)] = e;
   }
 }
@@ -675,6 +817,7 @@ struct fim_ctx {
   float *d_phi = nullptr, *d_rscale = nullptr;
   uint8_t *d_dirty = nullptr, *d_chg = nullptr;
   uint32_t *d_act = nullptr, *d_list[3] = {nullptr, nullptr, nullptr}, *d_cnt = nullptr;
+  float *d_wake = nullptr;
   // compat state
   double *d_phi64 = nullptr;
   int8_t *d_st = nullptr;
@@ -700,24 +843,36 @@ struct fim_ctx {
 
 namespace {
 
-void launch_conv_compute(fim_ctx *c, const ConvArgs &a) {
+template <bool STAGED>
+void launch_conv_compute_t(fim_ctx *c, const ConvArgs &a) {
   switch (c->block) {
-    case 64: k_conv_compute<64><<<c->grid, 64, c->lds_conv, c->stream>>>(a); break;
-    case 128: k_conv_compute<128><<<c->grid, 128, c->lds_conv, c->stream>>>(a); break;
-    case 512: k_conv_compute<512><<<c->grid, 512, c->lds_conv, c->stream>>>(a); break;
-    default: k_conv_compute<256><<<c->grid, 256, c->lds_conv, c->stream>>>(a); break;
+    case 64: k_conv_compute<64, STAGED><<<c->grid, 64, c->lds_conv, c->stream>>>(a); break;
+    case 128: k_conv_compute<128, STAGED><<<c->grid, 128, c->lds_conv, c->stream>>>(a); break;
+    case 512: k_conv_compute<512, STAGED><<<c->grid, 512, c->lds_conv, c->stream>>>(a); break;
+    default: k_conv_compute<256, STAGED><<<c->grid, 256, c->lds_conv, c->stream>>>(a); break;
   }
+}
+
+void launch_conv_compute(fim_ctx *c, const ConvArgs &a) {
+  if (c->o.staged)
+    launch_conv_compute_t<true>(c, a);
+  else
+    launch_conv_compute_t<false>(c, a);
   HIPCHECK(hipGetLastError());
 }
 
-void set_lds_attr(fim_ctx *c) {
-  const void *f = nullptr;
-  switch (c->block) {
-    case 64: f = (const void *)k_conv_compute<64>; break;
-    case 128: f = (const void *)k_conv_compute<128>; break;
-    case 512: f = (const void *)k_conv_compute<512>; break;
-    default: f = (const void *)k_conv_compute<256>; break;
+template <bool STAGED>
+const void *conv_kernel_ptr(int block) {
+  switch (block) {
+    case 64: return (const void *)k_conv_compute<64, STAGED>;
+    case 128: return (const void *)k_conv_compute<128, STAGED>;
+    case 512: return (const void *)k_conv_compute<512, STAGED>;
+    default: return (const void *)k_conv_compute<256, STAGED>;
   }
+}
+
+void set_lds_attr(fim_ctx *c) {
+  const void *f = c->o.staged ? conv_kernel_ptr<true>(c->block) : conv_kernel_ptr<false>(c->block);
   HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_conv));
   const void *g = nullptr;
   switch (c->block) {
@@ -878,18 +1033,24 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
       if (o->mode == FIM_MODE_CONVERGE) {
         c->lds_conv = 0;
         const uint32_t nn64 = ((uint32_t)c->nn_max + 63u) & ~63u;
+        const uint32_t ne32 = ((uint32_t)c->ne_max + 31u) / 32u;
         const uint32_t o_phin = al16(4 * nn64);
         const uint32_t o_flg = o_phin + al16(4 * nn64);
         const uint32_t o_tch = o_flg + al16(4 * (nn64 / 32));
-        const uint32_t o_e2n = o_tch + al16(4 * (nn64 / 32));
-        const uint32_t o_geo = o_e2n + al16(8 * (uint32_t)c->ne_max);
-        const uint32_t o_list = o_geo + al16(24 * (uint32_t)c->ne_max);
-        const uint32_t o_misc = o_list + al16(2 * (uint32_t)c->ne_max);
+        const uint32_t o_dfr = o_tch + al16(4 * (nn64 / 32));
+        const uint32_t o_e2n = o_dfr + al16(4 * (nn64 / 32));
+        // staged: element nodes + geometry; unstaged: element mark bitmasks
+        const uint32_t o_geo = o_e2n + (o->staged ? al16(8 * (uint32_t)c->ne_max) : al16(4 * ne32));
+        const uint32_t o_list = o_geo + (o->staged ? al16(24 * (uint32_t)c->ne_max) : al16(4 * ne32));
+        const uint32_t o_vlist = o_list + al16(2 * (uint32_t)c->ne_max);
+        const uint32_t o_misc = o_vlist + (o->staged ? 0u : al16(2 * nn64));
         c->lds_conv = o_misc + 16;
         if (c->lds_conv > 160 * 1024)
           throw Error(FIM_ELIMIT, "largest submesh needs " + std::to_string(c->lds_conv) +
                                       " B of LDS (limit 163840); use a smaller partition budget");
         c->d_geo32 = dupload(p.geo32, ow);
+        c->d_n2e_dsp = dupload(p.n2e_dsp, ow);
+        c->d_n2e_le = dupload(p.n2e_le, ow);
         c->d_if_l = dupload(p.if_l, ow);
         c->d_if_dsp = dupload(p.if_dsp, ow);
         c->d_if_other = dupload(p.if_other, ow);
@@ -898,8 +1059,9 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         c->d_dirty = dalloc<uint8_t>(ncs, ow);
         c->d_chg = dalloc<uint8_t>(ncs, ow);
         c->d_act = dalloc<uint32_t>(nss, ow);
+        c->d_wake = dalloc<float>(nss, ow);
         for (int k = 0; k < 3; ++k) c->d_list[k] = dalloc<uint32_t>(nss, ow);
-        c->d_cnt = dalloc<uint32_t>(3, ow);
+        c->d_cnt = dalloc<uint32_t>(4, ow);  // three list counters + iteration
         c->d_rscale = dalloc<float>(4 * (size_t)c->ninst, ow);
         c->d_tmpf = dalloc<float>(c->nn, ow);
         k_fill<float><<<blocks_for(4 * (size_t)c->ninst, 256), 256, 0, c->stream>>>(c->d_rscale, 4 * (size_t)c->ninst, 1.0f);
@@ -909,13 +1071,14 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         c->grid = o->grid > 0 ? o->grid : ncu * std::min(per_cu, 2048 / c->block);
         c->lds_compat = 0;
         c->ca = ConvArgs{c->d_sub, c->d_e2n, c->d_geo32, c->d_ereg, c->d_rscale, c->d_phi,
-                         c->d_dirty, c->d_chg, c->d_act, nullptr, nullptr, nullptr, nullptr,
-                         nullptr, c->d_stats, c->nsub, c->ncopy,
-                         (float)(o->eps_rel > 0 ? o->eps_rel : 1e-6), o->local_iters,
-                         o_phin, o_flg, o_tch, o_e2n, o_geo, o_list, o_misc};
+                         c->d_dirty, c->d_chg, c->d_act, c->d_wake, nullptr, nullptr, nullptr,
+                         nullptr, nullptr, c->d_stats, c->d_cnt + 3, c->d_n2e_dsp, c->d_n2e_le,
+                         c->nsub, c->ncopy, (float)(o->eps_rel > 0 ? o->eps_rel : 1e-6),
+                         (float)o->window, o->local_iters, o_phin, o_flg, o_tch, o_dfr, o_e2n,
+                         o_geo, o_list, o_vlist, o_misc};
         c->sa = SyncArgs{c->d_sub, c->d_if_l, c->d_if_dsp, c->d_if_other, c->d_c2s, c->d_phi,
-                         c->d_dirty, c->d_chg, c->d_act, nullptr, nullptr, nullptr, nullptr,
-                         c->nsub, c->ncopy};
+                         c->d_dirty, c->d_chg, c->d_act, c->d_wake, nullptr, nullptr, nullptr,
+                         nullptr, c->d_cnt + 3, c->nsub, c->ncopy};
       } else {
         const uint32_t nn = (uint32_t)c->nn_max;
         const uint32_t o_tphi = al16(8 * nn);
@@ -972,7 +1135,8 @@ int fim_reset(fim_ctx *c) {
       k_fill<uint8_t><<<blocks_for(ncs, 256), 256, 0, c->stream>>>(c->d_dirty, ncs, (uint8_t)0);
       k_fill<uint8_t><<<blocks_for(ncs, 256), 256, 0, c->stream>>>(c->d_chg, ncs, (uint8_t)0);
       k_fill<uint32_t><<<blocks_for(nss, 256), 256, 0, c->stream>>>(c->d_act, nss, 0u);
-      k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_cnt, 3, 0u);
+      k_fill<float><<<blocks_for(nss, 256), 256, 0, c->stream>>>(c->d_wake, nss, F_INF);
+      k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_cnt, 4, 0u);
     } else {
       k_fill<double><<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_phi64, c->ncopy, DBL_MAX);
       k_fill<int8_t><<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_st, c->ncopy, (int8_t)FIM_FAR);
@@ -998,7 +1162,8 @@ int fim_seed(fim_ctx *c, int32_t inst, const int64_t *vtx, int64_t n, double t0)
         if (!(t0 >= 0.0)) throw Error(FIM_EINVAL, "converge mode needs t0 >= 0");
         k_conv_seed<<<g, 256, 0, c->stream>>>(d, (uint32_t)cp.size(), (uint32_t)inst, (float)t0,
                                               c->d_c2s, c->d_phi, c->d_dirty, c->d_act,
-                                              c->d_list[0], c->d_cnt, c->nsub, c->ncopy);
+                                              c->d_wake, c->d_list[0], c->d_cnt, c->nsub,
+                                              c->ncopy);
       } else {
         k_compat_seed<<<g, 256, 0, c->stream>>>(d, (uint32_t)cp.size(), t0, c->d_phi64, c->d_st);
       }
@@ -1040,7 +1205,7 @@ int fim_init_field(fim_ctx *c, int32_t inst, const float *phi0) {
     k_conv_init<<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_tmpf, c->d_c2g, (uint32_t)inst,
                                                                    c->d_phi, c->d_dirty, c->ncopy);
     k_conv_queue_all<<<blocks_for(c->nsub, 256), 256, 0, c->stream>>>((uint32_t)inst, c->nsub, c->d_act,
-                                                                       c->d_list[0], c->d_cnt);
+                                                                       c->d_wake, c->d_list[0], c->d_cnt);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(c->stream));
   });
@@ -1124,6 +1289,7 @@ int fim_run(fim_ctx *c, int32_t max_iters, int32_t *iters) {
     c->stats.elem_updates = (int64_t)st[2];
     c->stats.global_iters = (int64_t)st[3];
     c->stats.alg_bytes = (int64_t)st[4];
+    c->stats.vertex_updates = (int64_t)st[5];
     c->stats.compute_ms = kms;
     c->stats.compute_launches = klaunch;
     if (iters) *iters = (int32_t)st[3];

@@ -54,13 +54,15 @@ class Opts(C.Structure):
     _fields_ = [("mode", C.c_int32), ("local", C.c_int32), ("n_instances", C.c_int32),
                 ("device", C.c_int32), ("block", C.c_int32), ("local_iters", C.c_int32),
                 ("graph_iters", C.c_int32), ("grid", C.c_int32), ("eps_rel", C.c_double),
-                ("apd", C.c_double), ("profile", C.c_int32), ("reserved", C.c_int32)]
+                ("apd", C.c_double), ("profile", C.c_int32), ("staged", C.c_int32),
+                ("window", C.c_double)]
 
 
 class Stats(C.Structure):
     _fields_ = [("global_iters", C.c_int64), ("activations", C.c_int64),
                 ("local_iters", C.c_int64), ("elem_updates", C.c_int64), ("last_ms", C.c_double),
-                ("alg_bytes", C.c_int64), ("compute_launches", C.c_int64), ("compute_ms", C.c_double)]
+                ("alg_bytes", C.c_int64), ("compute_launches", C.c_int64), ("compute_ms", C.c_double),
+                ("vertex_updates", C.c_int64)]
 
 
 # every symbol include/cufim.h declares, with its ctypes signature
@@ -221,12 +223,12 @@ class Pack:
 class Solver:
     def __init__(self, pack: Pack, mode=MODE_CONVERGE, local=LOCAL_EXACT, n_instances=1, device=0,
                  block=256, local_iters=0, graph_iters=0, grid=0, eps_rel=0.0, apd=0.0,
-                 profile=False):
+                 profile=False, window=0.0, staged=False):
         self.pack = pack
         self.nn = pack.info["nn_global"]
         self.n_instances = n_instances
         o = Opts(mode, local, n_instances, device, block, local_iters, graph_iters, grid, eps_rel, apd,
-                 int(profile), 0)
+                 int(profile), int(staged), window)
         self.h = _vp()
         _check(lib().fim_create(pack.h, C.byref(o), C.byref(self.h)))
 

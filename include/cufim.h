@@ -163,9 +163,18 @@ typedef struct {
   int32_t grid;          /* workgroups of the persistent kernels (0 = auto)  */
   double eps_rel;        /* convergence threshold (0 = 1e-6, reference)      */
   double apd;            /* compat: action potential duration, us (0 = 3e5)  */
-  int32_t profile;       /* converge: time every compute-kernel launch with
-                            HIP events recorded inside the graph            */
-  int32_t reserved;
+  int32_t profile;       /* converge: launch iterations without the graph and
+                            time every compute-kernel launch with events    */
+  int32_t staged;        /* converge: 1 = copy each activated submesh's
+                            elements into LDS and scan them every sweep;
+                            0 = elements stay in global memory and the
+                            active ones are found from flagged vertices     */
+  double window;         /* converge: time window (us) added per global
+                            iteration; a flagged vertex later than
+                            (iteration+1)*window is lowered but does not act
+                            as a source until the window passes it (0 = off,
+                            plain FIM).  The fixed point does not depend on
+                            it.                                             */
 } fim_opts;
 
 typedef struct {
@@ -179,6 +188,7 @@ typedef struct {
                                interface copy of each activated submesh     */
   int64_t compute_launches; /* compute-kernel launches timed (profile)      */
   double compute_ms;        /* summed compute-kernel time (profile)         */
+  int64_t vertex_updates;   /* (element, target vertex) candidates computed */
 } fim_stats_t;
 
 int fim_create(const fim_pack *p, const fim_opts *o, fim_ctx **out);

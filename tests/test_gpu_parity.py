@@ -125,13 +125,14 @@ def test_converge_anisotropic_slab_vs_fp64_oracle():
     _check_fp32(phi, ref)
 
 
+@pytest.mark.parametrize("staged", [False, True])
 @pytest.mark.parametrize("block", [64, 128, 256, 512])
-def test_converge_workgroup_shapes_agree(block):
+def test_converge_workgroup_shapes_agree(block, staged):
     cfg = cases.scaled(cases.C2, 8)
     m, om = cases.box(cfg, seed=5)
     p = F.Pack(m, F.PART_RCB, max_nn=256, seed=5)
     src = cases.pacing_sites(cfg, 3, seed=5)
-    phi, *_ = _converge(m, p, src, block=block)
+    phi, *_ = _converge(m, p, src, block=block, staged=staged)
     ref, _ = O.fim_solve(om, src, variant=O.LOCAL_EXACT)
     _check_fp32(phi, ref)
 
@@ -152,3 +153,55 @@ def test_converge_deterministic_and_batched():
         phi, _ = b.download(inst=i)
         assert np.array_equal(phi, singles[i])  # batching changes nothing
     b.close()
+
+
+@pytest.mark.parametrize("staged", [False, True])
+@pytest.mark.parametrize("window", [300.0, 1000.0, 4000.0])
+def test_converge_time_window_same_fixed_point(window, staged):
+    """the per-iteration time window changes the processing order only"""
+    cfg = cases.scaled(cases.C2, 4)
+    m, om = cases.box(cfg, seed=2)
+    p = F.Pack(m, F.PART_RCB, max_nn=512, seed=2)
+    src = cases.pacing_sites(cfg, 5, seed=2)
+    phi, st, it, res, stats = _converge(m, p, src, window=window, staged=staged)
+    ref, _ = O.fim_solve(om, src, variant=O.LOCAL_EXACT)
+    _check_fp32(phi, ref)
+    assert res < 1e-5
+
+
+def test_warm_start_from_upper_bound_reaches_same_fixed_point():
+    """init_field with every vertex active: an upper bound (oracle + 5 %)
+    converges to the cold-start fixed point"""
+    cfg = cases.scaled(cases.C2, 8)
+    m, om = cases.box(cfg, seed=9)
+    p = F.Pack(m, F.PART_RCB, max_nn=256, seed=9)
+    src = cases.pacing_sites(cfg, 2, seed=9)
+    cold, *_ = _converge(m, p, src)
+    ref, _ = O.fim_solve(om, src, variant=O.LOCAL_EXACT)
+    s = F.Solver(p)
+    s.seed(src, 0.0)
+    s.init_field((ref * 1.05 + 100.0).astype(np.float32))
+    s.seed(src, 0.0)
+    s.run()
+    warm, _ = s.download()
+    s.close()
+    _check_fp32(warm, ref)
+    assert np.max(np.abs(warm - cold)) <= 1e-4 * ref.max()
+
+
+def test_regional_scaling_equals_scaled_mesh():
+    """one instance with all four regional factors s equals the mesh with
+    both speeds multiplied by s (times divide by s)"""
+    cfg = cases.scaled(cases.C1, 4)
+    m, om = cases.box(cfg, seed=4)
+    p = F.Pack(m, F.PART_RCB, max_nn=128, seed=4)
+    s = F.Solver(p, n_instances=2)
+    s.seed([0], 0.0, inst=0)
+    s.seed([0], 0.0, inst=1)
+    s.set_scaling(np.full(4, 1.25, np.float32), inst=1)
+    s.run()
+    a, _ = s.download(inst=0)
+    b, _ = s.download(inst=1)
+    s.close()
+    fin = a < 1e50
+    assert np.allclose(b[fin] * 1.25, a[fin], rtol=2e-6, atol=1e-3)
