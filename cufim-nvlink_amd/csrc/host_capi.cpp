@@ -5,6 +5,7 @@
 #include <string>
 
 #include "host.hpp"
+#include "solver_fim.hpp"
 
 namespace fim {
 namespace {
@@ -208,5 +209,61 @@ int fim_pack_partition(const fim_pack *p, int64_t *e2s, int64_t *c2g) {
 }
 
 void fim_pack_destroy(fim_pack *p) { delete p; }
+
+int fim_config_load_options(const char *cfg_path, fim_options_t *out) {
+  return guarded([&] {
+    if (!cfg_path || !out) throw Error(FIM_EINVAL, "null argument");
+    const Options o = load_options(cfg_path);
+    *out = fim_options_t{o.verbose, o.method_id, o.solvetype, o.rent_en, o.dt, o.n_steps};
+  });
+}
+
+int fim_config_load_plan(const char *plan_path, fim_options_t *opts, int32_t *n_events,
+                         int64_t *n_vtx) {
+  return guarded([&] {
+    if (!plan_path || !opts) throw Error(FIM_EINVAL, "null argument");
+    Options o;
+    o.dt = opts->dt;
+    o.n_steps = opts->n_steps;
+    const Plan p = load_plan(plan_path, o);
+    opts->dt = o.dt;
+    opts->n_steps = o.n_steps;
+    int64_t nv = 0;
+    for (const auto &e : p.events) nv += (int64_t)e.vtx.size();
+    if (n_events) *n_events = (int32_t)p.events.size();
+    if (n_vtx) *n_vtx = nv;
+  });
+}
+
+int fim_config_plan_events(const char *plan_path, int32_t *active, double *tstart, double *dur,
+                           int64_t *cnt, int64_t *vtx) {
+  return guarded([&] {
+    if (!plan_path) throw Error(FIM_EINVAL, "null argument");
+    Options o;
+    const Plan p = load_plan(plan_path, o);
+    int64_t k = 0;
+    for (size_t i = 0; i < p.events.size(); ++i) {
+      if (active) active[i] = p.events[i].active;
+      if (tstart) tstart[i] = p.events[i].tstart;
+      if (dur) dur[i] = p.events[i].dur;
+      if (cnt) cnt[i] = (int64_t)p.events[i].vtx.size();
+      for (int64_t v : p.events[i].vtx) {
+        if (vtx) vtx[k] = v;
+        ++k;
+      }
+    }
+  });
+}
+
+int fim_run_reference(const char *mesh_path, const char *part_path, const char *plan_path,
+                      const char *out_path, int32_t num_reps, const char *cfg_path,
+                      int32_t device, double *avg_s) {
+  return guarded([&] {
+    if (!mesh_path || !part_path || !plan_path) throw Error(FIM_EINVAL, "null argument");
+    const double s = run(mesh_path, part_path, plan_path, out_path ? out_path : "", num_reps,
+                         cfg_path ? cfg_path : "data/default.cfg", device);
+    if (avg_s) *avg_s = s;
+  });
+}
 
 }  // extern "C"

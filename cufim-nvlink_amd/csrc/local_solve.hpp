@@ -20,6 +20,27 @@
 
 namespace fim {
 
+// square root and quotient of the exact candidates.  FP64 (and host code)
+// uses the correctly rounded operations; the FP32 device path uses the
+// hardware square root and reciprocal (1 ulp), which is far below the FP32
+// solver's error budget.
+FIM_HD inline double csqrt(double x) { return std::sqrt(x); }
+FIM_HD inline double cdiv(double a, double b) { return a / b; }
+FIM_HD inline float csqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return std::sqrt(x);
+#endif
+}
+FIM_HD inline float cdiv(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return a * __builtin_amdgcn_rcpf(b);
+#else
+  return a / b;
+#endif
+}
+
 // index of the edge between node positions i, j in the 6-entry geometry
 FIM_HD constexpr int edge_idx(int i, int j) {
   return (i > j) ? edge_idx(j, i) : (j == 3 ? 3 + i : i + j - 1);
@@ -40,10 +61,10 @@ FIM_HD inline T tri_exact(T e12, T e14, T e24, T p1, T p2, T inf) {
   const T G = e12, g = T(0.5) * (e12 + e14 - e24), d = p2 - p1;
   const T den = G - d * d;
   if (!(den > T(0))) return inf;
-  const T r = e14 - g * g / G;
+  const T r = e14 - cdiv(g * g, G);
   if (!(r > T(0))) return inf;
-  const T s = std::sqrt(r * G / den);
-  const T l = (g - s * d) / G;
+  const T s = csqrt(cdiv(r * G, den));
+  const T l = cdiv(g - s * d, G);
   if (!(l > T(0) && l < T(1))) return inf;
   return p1 + l * d + s;
 }
@@ -56,14 +77,14 @@ FIM_HD inline T tet_exact(T e12, T e13, T e23, T e14, T e24, T e34, T p1, T p2, 
   const T d1 = p1 - p3, d2 = p2 - p3;
   const T det = AA * BB - AB * AB;
   if (!(det > T(0))) return inf;
-  const T w = (BB * d1 * d1 - T(2) * AB * d1 * d2 + AA * d2 * d2) / det;
+  const T w = cdiv(BB * d1 * d1 - T(2) * AB * d1 * d2 + AA * d2 * d2, det);
   if (!(w < T(1))) return inf;
-  const T r = e34 - (BB * Au * Au - T(2) * AB * Au * Bu + AA * Bu * Bu) / det;
+  const T r = e34 - cdiv(BB * Au * Au - T(2) * AB * Au * Bu + AA * Bu * Bu, det);
   if (!(r > T(0))) return inf;
-  const T s = std::sqrt(r / (T(1) - w));
+  const T s = csqrt(cdiv(r, T(1) - w));
   const T t1 = Au - s * d1, t2 = Bu - s * d2;
-  const T l1 = (BB * t1 - AB * t2) / det;
-  const T l2 = (AA * t2 - AB * t1) / det;
+  const T l1 = cdiv(BB * t1 - AB * t2, det);
+  const T l2 = cdiv(AA * t2 - AB * t1, det);
   if (!(l1 > T(0) && l2 > T(0) && l1 + l2 < T(1))) return inf;
   return p3 + l1 * d1 + l2 * d2 + s;
 }
@@ -77,9 +98,9 @@ FIM_HD inline T cand_exact(const T g[6], const T ph[4], T inf) {
   const T p1 = ph[q1], p2 = ph[q2], p3 = ph[q3];
   const bool u1 = p1 < inf, u2 = p2 < inf, u3 = p3 < inf;
   T best = inf;
-  if (u1) best = std::fmin(best, p1 + std::sqrt(e14));
-  if (u2) best = std::fmin(best, p2 + std::sqrt(e24));
-  if (u3) best = std::fmin(best, p3 + std::sqrt(e34));
+  if (u1) best = std::fmin(best, p1 + csqrt(e14));
+  if (u2) best = std::fmin(best, p2 + csqrt(e24));
+  if (u3) best = std::fmin(best, p3 + csqrt(e34));
   if (u1 && u2) best = std::fmin(best, tri_exact<T>(e12, e14, e24, p1, p2, inf));
   if (u1 && u3) best = std::fmin(best, tri_exact<T>(e13, e14, e34, p1, p3, inf));
   if (u2 && u3) best = std::fmin(best, tri_exact<T>(e23, e24, e34, p2, p3, inf));

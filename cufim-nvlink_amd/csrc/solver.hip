@@ -203,8 +203,8 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
           bool on = false;
           if (e < ne) {
             const uint2 q = el[e];
-            on = flag_at(flg, q.x & 0xffffu) | flag_at(flg, q.x >> 16) |
-                 flag_at(flg, q.y & 0xffffu) | flag_at(flg, q.y >> 16);
+            on = flag_at(flg, q.x & 0xffffu) || flag_at(flg, q.x >> 16) ||
+                 flag_at(flg, q.y & 0xffffu) || flag_at(flg, q.y >> 16);
           }
           const uint64_t m = __ballot(on);
           uint32_t base = 0;

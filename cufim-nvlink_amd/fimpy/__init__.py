@@ -58,6 +58,11 @@ class Opts(C.Structure):
                 ("window", C.c_double)]
 
 
+class CfgOptions(C.Structure):
+    _fields_ = [("verbose", C.c_int32), ("method_id", C.c_int32), ("solvetype", C.c_int32),
+                ("rent_en", C.c_int32), ("dt", C.c_double), ("n_steps", C.c_int64)]
+
+
 class Stats(C.Structure):
     _fields_ = [("global_iters", C.c_int64), ("activations", C.c_int64),
                 ("local_iters", C.c_int64), ("elem_updates", C.c_int64), ("last_ms", C.c_double),
@@ -98,6 +103,11 @@ SIGNATURES = {
     "fim_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
     "fim_destroy": (None, [_vp]),
     "fim_device_info": (C.c_int, [C.c_int32, C.c_char_p, C.c_int64]),
+    "fim_config_load_options": (C.c_int, [C.c_char_p, C.POINTER(CfgOptions)]),
+    "fim_config_load_plan": (C.c_int, [C.c_char_p, C.POINTER(CfgOptions), _pi32, _pi64]),
+    "fim_config_plan_events": (C.c_int, [C.c_char_p, _pi32, _pd, _pd, _pi64, _pi64]),
+    "fim_run_reference": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32,
+                                    C.c_char_p, C.c_int32, _pd]),
 }
 
 _LIB = None
@@ -288,3 +298,37 @@ def device_info(device=0):
     buf = C.create_string_buffer(512)
     _check(lib().fim_device_info(device, buf, 512))
     return buf.value.decode()
+
+
+def load_options(path):
+    o = CfgOptions()
+    _check(lib().fim_config_load_options(path.encode(), C.byref(o)))
+    return {k: getattr(o, k) for k, _ in CfgOptions._fields_}
+
+
+def load_plan(path, opts=None):
+    """Returns (options with the plan's dt / n_steps, events)."""
+    o = CfgOptions(**opts) if opts else CfgOptions()
+    ne, nv = C.c_int32(), C.c_int64()
+    _check(lib().fim_config_load_plan(path.encode(), C.byref(o), C.byref(ne), C.byref(nv)))
+    act = np.zeros(ne.value, np.int32)
+    ts = np.zeros(ne.value)
+    du = np.zeros(ne.value)
+    cnt = np.zeros(ne.value, np.int64)
+    vtx = np.zeros(nv.value, np.int64)
+    _check(lib().fim_config_plan_events(path.encode(), _p(act, _pi32), _p(ts, _pd), _p(du, _pd),
+                                        _p(cnt, _pi64), _p(vtx, _pi64)))
+    events, k = [], 0
+    for i in range(ne.value):
+        events.append(dict(active=int(act[i]), tstart=float(ts[i]), dur=float(du[i]),
+                           vtx=vtx[k:k + cnt[i]].copy()))
+        k += cnt[i]
+    return {k: getattr(o, k) for k, _ in CfgOptions._fields_}, events
+
+
+def run_reference(mesh, part, plan, out, reps=1, cfg="data/default.cfg", device=0):
+    s = C.c_double()
+    _check(lib().fim_run_reference(mesh.encode(), part.encode(), plan.encode(),
+                                   out.encode() if out else b"", reps, cfg.encode(), device,
+                                   C.byref(s)))
+    return s.value

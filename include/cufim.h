@@ -223,6 +223,31 @@ int fim_residual(fim_ctx *c, int32_t inst, double *res);
 int fim_stats(const fim_ctx *c, fim_stats_t *s);
 void fim_destroy(fim_ctx *c);
 
+/* ========================================================================
+ * Reference-format configuration and end-to-end run
+ * ===================================================================== */
+typedef struct {
+  int32_t verbose, method_id, solvetype, rent_en;
+  double dt;          /* microseconds */
+  int64_t n_steps;
+} fim_options_t;
+
+/* data/default.cfg (src/io/io_json.cpp:228-296) */
+int fim_config_load_options(const char *cfg_path, fim_options_t *out);
+/* plan JSON (src/io/io_json.cpp:299-318): overrides opts->dt / n_steps and
+ * reports the number of stimulus events and of their vertices */
+int fim_config_load_plan(const char *plan_path, fim_options_t *opts, int32_t *n_events,
+                         int64_t *n_vtx);
+/* the events of a plan: active, start (us), duration (us), vertex count and
+ * the concatenated vertex ids, in the reference's order */
+int fim_config_plan_events(const char *plan_path, int32_t *active, double *tstart, double *dur,
+                           int64_t *cnt, int64_t *vtx);
+/* tw::run (src/main.cpp:84-104): mesh + tags + cfg + plan -> compat solve
+ * -> VTK output; *avg_s = mean step-loop seconds */
+int fim_run_reference(const char *mesh_path, const char *part_path, const char *plan_path,
+                      const char *out_path, int32_t num_reps, const char *cfg_path,
+                      int32_t device, double *avg_s);
+
 /* device description (replaces cuda_info::print, src/core/cuda_base.cu:72-309) */
 int fim_device_info(int32_t device, char *buf, int64_t len);
 

@@ -1,0 +1,230 @@
+"""Host side of the product, CPU only: the C ABI exports, mesh / VTK / config
+parsing against the reference's own host code (oracle/_ref), the partitioner
+and error paths.  No GPU call is made here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases
+import fimpy as F
+import pyoracle as O
+import pyref as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+need_ref = pytest.mark.skipif(not R.available(), reason="oracle/_ref not built (needs reference sources)")
+UTV = f"{cases.UT}/ut_mesh.vtk"
+UTT = f"{cases.UT}/ut_mesh.tags"
+
+
+def test_library_exports_every_declared_symbol():
+    text = open(os.path.join(ROOT, "include", "cufim.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(fim_[a-z_0-9]+)\s*\(", text))
+    assert len(names) > 25
+    lib = ctypes.CDLL(F.LIB_PATH)
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert names == set(F.SIGNATURES), names ^ set(F.SIGNATURES)
+
+
+@need_ref
+def test_load_vtk_matches_reference():
+    m = F.Mesh.load_vtk(UTV)
+    xyz, e2n, tags = m.arrays()
+    rx, re2n, rtag = R.load_vtk(UTV)
+    R.connectivity(rx.shape[0], re2n.shape[0])
+    assert np.array_equal(xyz, rx)
+    rs, _ = R.connectivity(rx.shape[0], re2n.shape[0])
+    assert np.array_equal(e2n, rs)
+    assert np.array_equal(tags, rtag)
+
+
+@need_ref
+def test_save_vtk_bytes_match_reference(tmp_path):
+    m = F.Mesh.load_vtk(UTV)
+    rng = np.random.default_rng(0)
+    phi = rng.uniform(0, 5e5, m.nn)
+    phi[::7] = np.finfo(np.float64).max  # unreached -> -1
+    st = rng.integers(0, 6, m.nn).astype(np.int32)
+    ours = tmp_path / "ours.vtk"
+    theirs = tmp_path / "ref.vtk"
+    m.save_vtk(str(ours), phi, st)
+    R.load_vtk(UTV)
+    xyz, cells, _ = R.load_vtk(UTV)
+    R.connectivity(xyz.shape[0], cells.shape[0])
+    R.save_vtk(str(theirs), phi, st.astype(np.float64))
+    assert ours.read_bytes() == theirs.read_bytes()
+
+
+def test_save_vtk_format_of_golden():
+    """the reference's output file is reproduced byte for byte from its own
+    phi / states: the writer alone is pinned by the golden"""
+    import vtkio
+    g = vtkio.read_vtk(f"{cases.UT}/cuda_fim_res.vtk")
+    m = F.Mesh.load_vtk(UTV)
+    phi = np.where(g["point_data"]["phi"] < 0, np.finfo(np.float64).max, g["point_data"]["phi"])
+    st = g["point_data"]["states"].astype(np.int32)
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "o.vtk")
+        m.save_vtk(p, phi, st)
+        assert open(p).read() == open(f"{cases.UT}/cuda_fim_res.vtk").read()
+
+
+@need_ref
+def test_tag_partition_numbering_matches_reference():
+    for tagfile in ("ut_mesh.tags", "ut_mesh_p.tags"):
+        m = F.Mesh.load_vtk(UTV)
+        m.read_tags(f"{cases.UT}/{tagfile}")
+        p = F.Pack(m, F.PART_TAGS)
+        e2s, c2g = p.partition()
+        xyz, cells, _ = R.load_vtk(UTV)
+        R.set_mesh(xyz, cells, np.loadtxt(f"{cases.UT}/{tagfile}", dtype=np.int32))
+        R.connectivity(xyz.shape[0], cells.shape[0])
+        subs, _ = R.extract(xyz.shape[0])
+        assert np.array_equal(c2g, np.concatenate([s["l2g"] for s in subs]))
+
+
+@need_ref
+def test_options_and_plan_match_reference():
+    cfg = os.path.join(ROOT, "data", "default.cfg")
+    ours = F.load_options(cfg)
+    theirs = R.load_options(cfg)
+    for k in ("verbose", "solvetype", "rent_en", "dt", "n_steps"):
+        assert ours[k] == theirs[k], k
+    o2, ev = F.load_plan(f"{cases.UT}/ut_mesh.plan.json", ours)
+    rp = R.load_plan(f"{cases.UT}/ut_mesh.plan.json")
+    assert o2["dt"] == rp["dt"] and o2["n_steps"] == rp["n_steps"]
+    assert len(ev) == len(rp["events"])
+    for a, b in zip(ev, rp["events"]):
+        assert a["active"] == b["active"] and a["tstart"] == b["tstart"] and a["dur"] == b["dur"]
+        assert np.array_equal(a["vtx"], b["vtx"])
+
+
+def test_plan_pulse_trains_and_errors(tmp_path):
+    import json
+    plan = json.load(open(f"{cases.UT}/ut_mesh.plan.json"))
+    st = plan["simulation"]["stimuli"]["S0"]
+    st["protocol"].update(npls=3, bcl=400.0, start=5.0)
+    plan["simulation"]["stimuli"]["A_off"] = dict(st, active=0)
+    p = tmp_path / "p.json"
+    p.write_text(json.dumps(plan))
+    o, ev = F.load_plan(str(p))
+    assert o["dt"] == 150e3 and o["n_steps"] == 4
+    assert [e["tstart"] for e in ev] == [5e3, 405e3, 805e3]
+    plan["simulation"]["stimuli"]["S0"]["type"] = "Extracellular"
+    p.write_text(json.dumps(plan))
+    with pytest.raises(F.FimError, match="unknown stim type"):
+        F.load_plan(str(p))
+    del plan["simulation"]["stimuli"]
+    p.write_text(json.dumps(plan))
+    assert F.load_plan(str(p))[1] == []
+    with pytest.raises(F.FimError, match="FIM_EIO"):
+        F.load_plan(str(tmp_path / "missing.json"))
+    (tmp_path / "bad.cfg").write_text("true\nfmm\nvol\nconst\nmaybe\nfalse\nfrozen\n1.0\n10\n")
+    with pytest.raises(F.FimError, match="re_flag"):
+        F.load_options(str(tmp_path / "bad.cfg"))
+
+
+def test_rcb_partition_is_deterministic_and_valid():
+    m = F.Mesh.gen_box(seed=3, **cases.scaled(cases.C2, 4))
+    a = F.Pack(m, F.PART_RCB, max_nn=300, seed=11)
+    b = F.Pack(m, F.PART_RCB, max_nn=300, seed=11)
+    ea, ca = a.partition()
+    eb, cb = b.partition()
+    assert np.array_equal(ea, eb) and np.array_equal(ca, cb)
+    info = a.info
+    assert info["nn_max"] <= 300
+    assert np.bincount(ea).min() > 0 and ea.max() + 1 == info["nsub"]
+    # every vertex has a copy in each submesh holding one of its elements
+    _, e2n, _ = m.arrays()
+    for s in range(0, info["nsub"], 7):
+        verts = np.unique(e2n[ea == s])
+        assert len(verts) <= 300
+    c = F.Pack(m, F.PART_RCB, max_nn=300, seed=12)
+    assert c.info["nsub"] > 0
+
+
+def test_box_generator_is_seeded_and_conforming():
+    cfg = dict(cases.C1, nx=5, ny=4, nz=3)
+    a = F.Mesh.gen_box(seed=1, **cfg)
+    b = F.Mesh.gen_box(seed=1, **cfg)
+    c = F.Mesh.gen_box(seed=2, **cfg)
+    xa, ea, _ = a.arrays()
+    xb, eb, _ = b.arrays()
+    xc, _, _ = c.arrays()
+    assert np.array_equal(xa, xb) and np.array_equal(ea, eb) and not np.array_equal(xa, xc)
+    assert a.ne == 6 * 5 * 4 * 3 and a.nn == 6 * 5 * 4
+    # positive volumes
+    v = np.einsum("ij,ij->i", xa[ea[:, 1]] - xa[ea[:, 0]],
+                  np.cross(xa[ea[:, 2]] - xa[ea[:, 0]], xa[ea[:, 3]] - xa[ea[:, 0]]))
+    assert np.all(np.abs(v) > 0)
+    # conforming: every face shared by at most two elements, boundary faces
+    # exactly on the box surface
+    faces = {}
+    for t in ea:
+        for k in range(4):
+            f = tuple(sorted(np.delete(t, k)))
+            faces[f] = faces.get(f, 0) + 1
+    assert max(faces.values()) == 2
+    nb = sum(1 for v in faces.values() if v == 1)
+    assert nb == 2 * 2 * (5 * 4 + 4 * 3 + 5 * 3)
+    # jitter bound
+    h = np.array([cfg["lx"] / 5, cfg["ly"] / 4, cfg["lz"] / 3])
+    grid = np.stack(np.meshgrid(np.arange(6), np.arange(5), np.arange(4), indexing="ij"), -1)
+    grid = grid.transpose(2, 1, 0, 3).reshape(-1, 3) * h
+    assert np.all(np.abs(xa - grid) <= 0.1 * h + 1e-9)
+
+
+def test_geometry_matches_oracle_metric():
+    """product metric geometry (via the pack) agrees with the oracle's
+    closed form e^T M e, checked through orc_geo_elem on the same input"""
+    m = F.Mesh.gen_box(seed=5, **dict(cases.C2, nx=4, ny=4, nz=2))
+    xyz, e2n, _ = m.arrays()
+    fib, vel = m.material()
+    for e in range(0, e2n.shape[0], 5):
+        M = O.metric(fib[e], vel[e, 0], vel[e, 1])
+        g = O.geo_elem(xyz, e2n[e], M)
+        f = fib[e]
+        D = np.outer(f, f) / vel[e, 0] ** 2 + (np.eye(3) - np.outer(f, f)) / vel[e, 1] ** 2
+        x = xyz[e2n[e]]
+        direct = [(x[i] - x[j]) @ D @ (x[i] - x[j]) for i, j in ((0, 1), (0, 2), (1, 2), (0, 3), (1, 3), (2, 3))]
+        assert np.allclose(g, direct, rtol=1e-12)
+
+
+def test_error_paths_return_codes():
+    with pytest.raises(F.FimError, match="FIM_EIO"):
+        F.Mesh.load_vtk("/nonexistent.vtk")
+    with pytest.raises(F.FimError, match="FIM_EINVAL"):
+        F.Mesh.from_arrays(np.zeros((4, 3)), np.array([[0, 1, 2, 9]]))
+    with pytest.raises(F.FimError, match="repeated node"):
+        F.Mesh.from_arrays(np.eye(4, 3), np.array([[0, 1, 2, 2]]))
+    m = F.Mesh.load_vtk(UTV)
+    with pytest.raises(F.FimError, match="no partition tags"):
+        F.Pack(m, F.PART_TAGS)
+    with pytest.raises(F.FimError, match="non-integer token"):
+        m.read_tags(f"{cases.GOLDEN}/ut_mesh/ut_mesh.plan.json")
+    import tempfile
+    with tempfile.NamedTemporaryFile("w", suffix=".tags") as f:
+        f.write("1\n2\n3\n")
+        f.flush()
+        with pytest.raises(F.FimError, match="entries"):
+            m.read_tags(f.name)
+
+
+def test_cli_usage_and_arguments():
+    cli = F.CLI_PATH
+    r = subprocess.run([cli, "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "--mesh" in r.stdout
+    r = subprocess.run([cli, "--reps=0", "--mesh=a", "--part=b", "--plan=c", "--odir=d"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0 and "invalid number of reps!" in r.stdout
+    r = subprocess.run([cli, "--bogus"], capture_output=True, text=True)
+    assert r.returncode != 0
+    r = subprocess.run([cli, "--mesh=/nonexistent.vtk", "--part=x", "--plan=y", "--odir=z"],
+                       capture_output=True, text=True, cwd=ROOT)
+    assert r.returncode != 0 and "file not found" in r.stderr
