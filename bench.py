@@ -229,7 +229,7 @@ def main():
         "activations_per_step": tot["activations"] / args.steps,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_conv_compute",
+            "kernel": "k_conv_compute" if args.staged else "k_conv_band",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

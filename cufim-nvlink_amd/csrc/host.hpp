@@ -89,6 +89,8 @@ struct Pack {
   std::vector<SubInfo> sub;        // nsub
   std::vector<uint16_t> e2n_l;     // 4 per element: local copy ids, sorted
   std::vector<float> geo32;        // 6 per element
+  std::vector<float> hgt32;        // 4 per element: metric height of each node
+                                   // above its opposite face, rounded down
   std::vector<double> geo64;       // 6 per element
   std::vector<uint8_t> ereg;       // 1 per element
   std::vector<uint32_t> c2g;       // per copy

@@ -181,6 +181,7 @@ Pack build_pack(const Mesh &m, const Partition &p, int32_t source) {
   k.e2n_l.resize(4 * m.ne);
   k.geo32.resize(6 * m.ne);
   k.geo64.resize(6 * m.ne);
+  k.hgt32.resize(4 * m.ne);
   k.ereg.assign(m.ne, 0);
   k.c2g.resize(k.ncopy);
   k.c2s.resize(k.ncopy);
@@ -224,6 +225,26 @@ Pack build_pack(const Mesh &m, const Partition &p, int32_t source) {
                  (M[2] * x + M[4] * y + M[5] * z) * z;
         k.geo64[6 * i + q] = geo[q];
         k.geo32[6 * i + q] = (float)geo[q];
+      }
+      // heights: for target t with the others relabelled 1,2,3 (A = x1-x3,
+      // B = x2-x3, u = x4-x3), h^2 = |u|^2 - g^T G^-1 g with the Gram matrix G
+      // of (A, B) and g = (A.u, B.u).  Every candidate of t is at least
+      // min(other times) + h, which lets the kernel skip hopeless targets.
+      for (int t = 0; t < 4; ++t) {
+        const int q1 = (t + 1) & 3, q2 = (t + 2) & 3, q3 = (t + 3) & 3;
+        auto E = [&](int a, int b) { return geo[(a > b) ? ((a == 3) ? 3 + b : a + b - 1)
+                                                        : ((b == 3) ? 3 + a : a + b - 1)]; };
+        const double e12 = E(q1, q2), e13 = E(q1, q3), e23 = E(q2, q3);
+        const double e14 = E(q1, t), e24 = E(q2, t), e34 = E(q3, t);
+        const double AA = e13, BB = e23, AB = 0.5 * (e13 + e23 - e12);
+        const double Au = 0.5 * (e13 + e34 - e14), Bu = 0.5 * (e23 + e34 - e24);
+        const double det = AA * BB - AB * AB;
+        double h2 = 0.0;
+        if (det > 0.0) h2 = e34 - (BB * Au * Au - 2.0 * AB * Au * Bu + AA * Bu * Bu) / det;
+        // relative safety margin far above the rounding of h2 for any tet
+        // that is not a sliver; slivers get h = 0 (no pruning)
+        const double hh = h2 > 1e-6 * e34 ? std::sqrt(h2) * (1.0 - 1e-4) : 0.0;
+        k.hgt32[4 * i + t] = (float)hh;
       }
       if (!m.region.empty()) k.ereg[i] = m.region[e];
     }

@@ -62,6 +62,7 @@ struct ConvArgs {
   const SubInfo *sub;
   const uint2 *e2n;      // 4 x u16 local ids per element
   const float *geo;      // 6 per element
+  const float4 *hgt;     // per element: height of each node above its opposite face
   const uint8_t *ereg;   // region per element
   const float *rscale;   // n_inst x 4 geometry multipliers (1/s^2)
   float *phi;            // n_inst x ncopy
@@ -86,7 +87,7 @@ struct ConvArgs {
   float window;          // per-iteration time window, 0 = off
   int32_t max_local;
   // LDS carve (bytes)
-  uint32_t o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc;
+  uint32_t o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc, o_wrt;
 };
 
 // One workgroup per work item (instance, submesh).  STAGED copies the
@@ -294,24 +295,29 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
           f[k] = flag_at(flg, nd[k]) ? 1 : 0;
         }
         const int nf = f[0] + f[1] + f[2] + f[3];
-        // a target can improve only through a flagged other node and only
-        // if it is later than the earliest other node
-        if (nf - f[0] > 0 && ph[0] > fminf(ph[1], fminf(ph[2], ph[3]))) {
+        // every candidate of a target is at least the earliest other node
+        // plus the target's metric height above the opposite face: evaluate
+        // only targets with a flagged other node that can still improve
+        const size_t gh = (size_t)si.eoff + e;
+        const float hs = csqrt(rs[a.ereg[gh] & 3]);
+        const float4 hh = a.hgt[gh];
+        const float h[4] = {hh.x * hs, hh.y * hs, hh.z * hs, hh.w * hs};
+        if (nf - f[0] > 0 && ph[0] > fminf(ph[1], fminf(ph[2], ph[3])) + h[0]) {
           const float c = cand_exact<0, float>(g, ph, F_INF);
           ++vupd;
           if (c < ph[0]) atomicMin(&phi_n[nd[0]], __float_as_uint(c));
         }
-        if (nf - f[1] > 0 && ph[1] > fminf(ph[0], fminf(ph[2], ph[3]))) {
+        if (nf - f[1] > 0 && ph[1] > fminf(ph[0], fminf(ph[2], ph[3])) + h[1]) {
           const float c = cand_exact<1, float>(g, ph, F_INF);
           ++vupd;
           if (c < ph[1]) atomicMin(&phi_n[nd[1]], __float_as_uint(c));
         }
-        if (nf - f[2] > 0 && ph[2] > fminf(ph[0], fminf(ph[1], ph[3]))) {
+        if (nf - f[2] > 0 && ph[2] > fminf(ph[0], fminf(ph[1], ph[3])) + h[2]) {
           const float c = cand_exact<2, float>(g, ph, F_INF);
           ++vupd;
           if (c < ph[2]) atomicMin(&phi_n[nd[2]], __float_as_uint(c));
         }
-        if (nf - f[3] > 0 && ph[3] > fminf(ph[0], fminf(ph[1], ph[2]))) {
+        if (nf - f[3] > 0 && ph[3] > fminf(ph[0], fminf(ph[1], ph[2])) + h[3]) {
           const float c = cand_exact<3, float>(g, ph, F_INF);
           ++vupd;
           if (c < ph[3]) atomicMin(&phi_n[nd[3]], __float_as_uint(c));
@@ -364,7 +370,7 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
       __syncthreads();
       used = misc[3];
     }
-    abytes += 32ull * used + 10ull * nn + 12ull * si.ni;
+    abytes += 48ull * used + 10ull * nn + 12ull * si.ni;
     // write back decreased copies; remember them for the interface push.
     // Deferred vertices (and sources left by the sweep cap) stay dirty and
     // the submesh is queued for the next iteration.
@@ -400,6 +406,256 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
   for (int off = 32; off > 0; off >>= 1) vupd += __shfl_xor(vupd, off);
   if (lane == 0 && vupd) atomicAdd(&a.stats[5], vupd);
 }
+
+// expands the set bits of `bits` (word k of a bitmask) into list[base...]
+__device__ inline void expand_bits(uint32_t bits, uint32_t k, uint16_t *list, uint32_t base) {
+  while (bits) {
+    const uint32_t b = __ffs(bits) - 1;
+    bits &= bits - 1;
+    list[base++] = (uint16_t)(k * 32 + b);
+  }
+}
+
+// Unstaged converge kernel.  Per sweep it touches only the band that is
+// active: the flagged vertices mark their incident elements through the
+// vertex -> element adjacency (LDS bitmask), the marked elements are
+// evaluated from L2-resident data, LDS atomicMin winners set a "written"
+// bit, and only the written vertices are revisited to form the next
+// sweep's sources.  Element data never enters LDS.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_conv_band(ConvArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  float *phi_l = reinterpret_cast<float *>(lds);
+  uint32_t *phi_n = reinterpret_cast<uint32_t *>(lds + a.o_phin);
+  uint32_t *flg = reinterpret_cast<uint32_t *>(lds + a.o_flg);  // sources of this sweep
+  uint32_t *tch = reinterpret_cast<uint32_t *>(lds + a.o_tch);  // decreased this activation
+  uint32_t *dfr = reinterpret_cast<uint32_t *>(lds + a.o_dfr);  // flagged, beyond the window
+  uint32_t *wrt = reinterpret_cast<uint32_t *>(lds + a.o_wrt);  // lowered in this sweep
+  uint32_t *emark = reinterpret_cast<uint32_t *>(lds + a.o_e2n);
+  uint32_t *eseen = reinterpret_cast<uint32_t *>(lds + a.o_geo);
+  uint16_t *elist = reinterpret_cast<uint16_t *>(lds + a.o_list);
+  uint16_t *vlist = reinterpret_cast<uint16_t *>(lds + a.o_vlist);
+  uint32_t *misc = reinterpret_cast<uint32_t *>(lds + a.o_misc);
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+  const uint32_t n_work = *a.count;
+  if (blockIdx.x == 0 && tid == 0) {
+    *a.zero_count = 0;
+    if (n_work > 0) atomicAdd(&a.stats[3], 1ull);
+  }
+  unsigned long long sweeps = 0, evals = 0, acts = 0, abytes = 0, vupd = 0;
+  const float W = a.window > 0.0f ? a.window * (float)(*a.iter + 1u) : F_INF;
+
+  for (uint32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
+    const uint32_t entry = a.list[w];
+    const uint32_t inst = entry / a.nsub, s = entry - inst * a.nsub;
+    const SubInfo si = a.sub[s];
+    const size_t cb = (size_t)inst * a.ncopy + si.coff;
+    const uint32_t nn = si.nn, ne = si.ne;
+    const uint32_t nv64 = (nn + 63u) & ~63u;
+    const uint32_t nw = nv64 >> 5;           // vertex bitmask words
+    const uint32_t new32 = (ne + 31u) >> 5;  // element bitmask words
+    if (tid == 0) a.act[entry] = 0u;
+    if (a.wake[entry] > W) {
+      if (tid == 0 && atomicExch(&a.act[entry], 1u) == 0u) {
+        const uint32_t pos = atomicAdd(a.next_count, 1u);
+        a.next_list[pos] = entry;
+      }
+      continue;
+    }
+    // stage vertex times; dirty flags split into sources and deferred
+    for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
+      const uint32_t l = l0 + tid;
+      bool d = false, late = false;
+      if (l < nn) {
+        const float v = a.phi[cb + l];
+        phi_l[l] = v;
+        phi_n[l] = __float_as_uint(v);
+        d = a.dirty[cb + l] != 0;
+        if (d) a.dirty[cb + l] = 0;
+        late = d && v > W;
+        d = d && !late;
+      }
+      const uint64_t m = __ballot(d), md = __ballot(late);
+      const uint32_t wb = l0 + wave * 64;
+      if (wb < nv64) {
+        if (lane == 0) flg[wb >> 5] = (uint32_t)m;
+        if (lane == 1) flg[(wb >> 5) + 1] = (uint32_t)(m >> 32);
+        if (lane == 2) tch[wb >> 5] = 0u;
+        if (lane == 3) tch[(wb >> 5) + 1] = 0u;
+        if (lane == 4) dfr[wb >> 5] = (uint32_t)md;
+        if (lane == 5) dfr[(wb >> 5) + 1] = (uint32_t)(md >> 32);
+        if (lane == 6) wrt[wb >> 5] = 0u;
+        if (lane == 7) wrt[(wb >> 5) + 1] = 0u;
+      }
+    }
+    for (uint32_t k = tid; k < new32; k += BS) {
+      emark[k] = 0u;
+      eseen[k] = 0u;
+    }
+    if (tid == 0) misc[2] = 0u;
+    __syncthreads();
+    // initial source list
+    for (uint32_t k = tid; k < nw; k += BS) {
+      const uint32_t bits = flg[k];
+      if (bits) expand_bits(bits, k, vlist, atomicAdd(&misc[2], (uint32_t)__popc(bits)));
+    }
+    const float *rs = a.rscale + 4 * (size_t)inst;
+    __syncthreads();
+
+    int it = 0;
+    bool capped = false;
+    for (;;) {
+      const uint32_t nsrc = misc[2];
+      if (nsrc == 0) break;
+      if ((a.max_local > 0 && it >= a.max_local) || it >= kHardLocalCap) {
+        capped = true;
+        break;
+      }
+      ++it;
+      // mark the elements incident to the sources
+      for (uint32_t i = tid; i < nsrc; i += BS) {
+        const uint32_t c = si.coff + vlist[i];
+        const uint32_t kb = a.n2e_dsp[c], ke = a.n2e_dsp[c + 1];
+        for (uint32_t k = kb; k < ke; ++k) {
+          const uint32_t le = a.n2e_le[k];
+          atomicOr(&emark[le >> 5], 1u << (le & 31));
+        }
+      }
+      if (tid == 0) misc[0] = 0u;
+      __syncthreads();
+      // marked elements -> list (marks cleared)
+      for (uint32_t k = tid; k < new32; k += BS) {
+        const uint32_t bits = emark[k];
+        if (bits) {
+          emark[k] = 0u;
+          eseen[k] |= bits;
+          expand_bits(bits, k, elist, atomicAdd(&misc[0], (uint32_t)__popc(bits)));
+        }
+      }
+      __syncthreads();
+      const uint32_t nact = misc[0];
+      evals += nact;
+      // candidates of the marked elements
+      for (uint32_t i = tid; i < nact; i += BS) {
+        const uint32_t e = elist[i];
+        const size_t ge = (size_t)si.eoff + e;
+        const uint2 q = a.e2n[ge];
+        const float sc = rs[a.ereg[ge] & 3];
+        const float2 *g2 = reinterpret_cast<const float2 *>(a.geo + 6 * ge);
+        const float2 x0 = g2[0], x1 = g2[1], x2 = g2[2];
+        const float g[6] = {x0.x * sc, x0.y * sc, x1.x * sc, x1.y * sc, x2.x * sc, x2.y * sc};
+        const float hs = csqrt(sc);
+        const float4 hh = a.hgt[ge];
+        const float h[4] = {hh.x * hs, hh.y * hs, hh.z * hs, hh.w * hs};
+        const uint32_t nd[4] = {q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16};
+        float ph[4];
+        int f[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          ph[k] = phi_l[nd[k]];
+          f[k] = flag_at(flg, nd[k]) ? 1 : 0;
+        }
+        const int nf = f[0] + f[1] + f[2] + f[3];
+        // targets with a flagged other node whose bound (earliest other node
+        // + metric height) is still below their time
+#define FIM_TARGET(T, A, B, C)                                                        \
+        if (nf - f[T] > 0 && ph[T] > fminf(ph[A], fminf(ph[B], ph[C])) + h[T]) {         \
+          const float c = cand_exact<T, float>(g, ph, F_INF);                            \
+          ++vupd;                                                                         \
+          if (c < ph[T]) {                                                                \
+            const uint32_t cbits = __float_as_uint(c);                                    \
+            if (cbits < atomicMin(&phi_n[nd[T]], cbits))                                  \
+              atomicOr(&wrt[nd[T] >> 5], 1u << (nd[T] & 31));                             \
+          }                                                                               \
+        }
+        FIM_TARGET(0, 1, 2, 3)
+        FIM_TARGET(1, 0, 2, 3)
+        FIM_TARGET(2, 0, 1, 3)
+        FIM_TARGET(3, 0, 1, 2)
+#undef FIM_TARGET
+      }
+      if (tid == 0) misc[2] = 0u;
+      __syncthreads();
+      // accept the written vertices: significant decreases (reference
+      // criterion, cuda_fim.cu:2154) become the next sources or, beyond the
+      // window, deferred flags
+      for (uint32_t k = tid; k < nw; k += BS) {
+        uint32_t bits = wrt[k];
+        uint32_t sg = 0u, dc = 0u, lt = 0u;
+        if (bits) {
+          wrt[k] = 0u;
+          while (bits) {
+            const uint32_t b = __ffs(bits) - 1;
+            bits &= bits - 1;
+            const uint32_t l = k * 32 + b;
+            const float ov = phi_l[l];
+            const float nv = __uint_as_float(phi_n[l]);
+            if (nv < ov) {
+              phi_l[l] = nv;
+              dc |= 1u << b;
+              if ((ov - nv) >= a.eps * (1.0f + 0.5f * (ov + nv))) {
+                if (nv > W)
+                  lt |= 1u << b;
+                else
+                  sg |= 1u << b;
+              }
+            }
+          }
+          tch[k] |= dc;
+        }
+        flg[k] = sg;
+        dfr[k] = (dfr[k] & ~sg) | lt;
+        if (sg) expand_bits(sg, k, vlist, atomicAdd(&misc[2], (uint32_t)__popc(sg)));
+      }
+      __syncthreads();
+    }
+    sweeps += it;
+    acts += 1;
+    // algorithmic bytes: copies (phi 4+4, flag 1+1), interface entries
+    // (4+4+4) and the distinct elements used (nodes 8 + geometry 24 +
+    // heights 16)
+    if (tid == 0) misc[3] = 0u;
+    __syncthreads();
+    uint32_t cnt = 0;
+    for (uint32_t k = tid; k < new32; k += BS) cnt += __popc(eseen[k]);
+    if (cnt) atomicAdd(&misc[3], cnt);
+    if (tid == 0) misc[1] = __float_as_uint(F_INF);
+    __syncthreads();
+    abytes += 48ull * misc[3] + 10ull * nn + 12ull * si.ni;
+    // write back decreased copies; deferred vertices (and sources left by
+    // the sweep cap) stay dirty and the submesh is queued again
+    for (uint32_t l = tid; l < nn; l += BS) {
+      if (flag_at(tch, l)) {
+        a.phi[cb + l] = phi_l[l];
+        a.chg[cb + l] = 1;
+      }
+      if (flag_at(dfr, l) || (capped && flag_at(flg, l))) {
+        a.dirty[cb + l] = 1;
+        atomicMin(&misc[1], __float_as_uint(phi_l[l]));
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const float wk = __uint_as_float(misc[1]);
+      a.wake[entry] = wk;
+      if (wk < F_INF && atomicExch(&a.act[entry], 1u) == 0u) {
+        const uint32_t pos = atomicAdd(a.next_count, 1u);
+        a.next_list[pos] = entry;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && acts) {
+    atomicAdd(&a.stats[0], acts);
+    atomicAdd(&a.stats[1], sweeps);
+    atomicAdd(&a.stats[2], evals);
+    atomicAdd(&a.stats[4], abytes);
+  }
+  for (int off = 32; off > 0; off >>= 1) vupd += __shfl_xor(vupd, off);
+  if (lane == 0 && vupd) atomicAdd(&a.stats[5], vupd);
+}
+
 
 struct SyncArgs {
   const SubInfo *sub;
@@ -804,6 +1060,7 @@ struct fim_ctx {
   SubInfo *d_sub = nullptr;
   uint2 *d_e2n = nullptr;
   float *d_geo32 = nullptr;
+  float4 *d_hgt = nullptr;
   double *d_geo64 = nullptr;
   uint8_t *d_ereg = nullptr;
   uint32_t *d_c2g = nullptr, *d_c2s = nullptr, *d_g2c_dsp = nullptr, *d_g2c = nullptr, *d_grp = nullptr;
@@ -843,36 +1100,36 @@ struct fim_ctx {
 
 namespace {
 
-template <bool STAGED>
-void launch_conv_compute_t(fim_ctx *c, const ConvArgs &a) {
-  switch (c->block) {
-    case 64: k_conv_compute<64, STAGED><<<c->grid, 64, c->lds_conv, c->stream>>>(a); break;
-    case 128: k_conv_compute<128, STAGED><<<c->grid, 128, c->lds_conv, c->stream>>>(a); break;
-    case 512: k_conv_compute<512, STAGED><<<c->grid, 512, c->lds_conv, c->stream>>>(a); break;
-    default: k_conv_compute<256, STAGED><<<c->grid, 256, c->lds_conv, c->stream>>>(a); break;
-  }
-}
-
 void launch_conv_compute(fim_ctx *c, const ConvArgs &a) {
-  if (c->o.staged)
-    launch_conv_compute_t<true>(c, a);
-  else
-    launch_conv_compute_t<false>(c, a);
+  if (c->o.staged) {
+    switch (c->block) {
+      case 64: k_conv_compute<64, true><<<c->grid, 64, c->lds_conv, c->stream>>>(a); break;
+      case 128: k_conv_compute<128, true><<<c->grid, 128, c->lds_conv, c->stream>>>(a); break;
+      case 512: k_conv_compute<512, true><<<c->grid, 512, c->lds_conv, c->stream>>>(a); break;
+      default: k_conv_compute<256, true><<<c->grid, 256, c->lds_conv, c->stream>>>(a); break;
+    }
+  } else {
+    switch (c->block) {
+      case 64: k_conv_band<64><<<c->grid, 64, c->lds_conv, c->stream>>>(a); break;
+      case 128: k_conv_band<128><<<c->grid, 128, c->lds_conv, c->stream>>>(a); break;
+      case 512: k_conv_band<512><<<c->grid, 512, c->lds_conv, c->stream>>>(a); break;
+      default: k_conv_band<256><<<c->grid, 256, c->lds_conv, c->stream>>>(a); break;
+    }
+  }
   HIPCHECK(hipGetLastError());
 }
 
-template <bool STAGED>
-const void *conv_kernel_ptr(int block) {
+const void *conv_kernel_ptr(bool staged, int block) {
   switch (block) {
-    case 64: return (const void *)k_conv_compute<64, STAGED>;
-    case 128: return (const void *)k_conv_compute<128, STAGED>;
-    case 512: return (const void *)k_conv_compute<512, STAGED>;
-    default: return (const void *)k_conv_compute<256, STAGED>;
+    case 64: return staged ? (const void *)k_conv_compute<64, true> : (const void *)k_conv_band<64>;
+    case 128: return staged ? (const void *)k_conv_compute<128, true> : (const void *)k_conv_band<128>;
+    case 512: return staged ? (const void *)k_conv_compute<512, true> : (const void *)k_conv_band<512>;
+    default: return staged ? (const void *)k_conv_compute<256, true> : (const void *)k_conv_band<256>;
   }
 }
 
 void set_lds_attr(fim_ctx *c) {
-  const void *f = c->o.staged ? conv_kernel_ptr<true>(c->block) : conv_kernel_ptr<false>(c->block);
+  const void *f = conv_kernel_ptr(c->o.staged != 0, c->block);
   HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_conv));
   const void *g = nullptr;
   switch (c->block) {
@@ -1044,11 +1301,18 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         const uint32_t o_list = o_geo + (o->staged ? al16(24 * (uint32_t)c->ne_max) : al16(4 * ne32));
         const uint32_t o_vlist = o_list + al16(2 * (uint32_t)c->ne_max);
         const uint32_t o_misc = o_vlist + (o->staged ? 0u : al16(2 * nn64));
-        c->lds_conv = o_misc + 16;
+        const uint32_t o_wrt = o_misc + 16;
+        c->lds_conv = o_wrt + (o->staged ? 0u : al16(4 * (nn64 / 32)));
         if (c->lds_conv > 160 * 1024)
           throw Error(FIM_ELIMIT, "largest submesh needs " + std::to_string(c->lds_conv) +
                                       " B of LDS (limit 163840); use a smaller partition budget");
         c->d_geo32 = dupload(p.geo32, ow);
+        {
+          std::vector<float4> hg(p.ne_global);
+          for (int64_t e = 0; e < p.ne_global; ++e)
+            hg[e] = make_float4(p.hgt32[4 * e], p.hgt32[4 * e + 1], p.hgt32[4 * e + 2], p.hgt32[4 * e + 3]);
+          c->d_hgt = dupload(hg, ow);
+        }
         c->d_n2e_dsp = dupload(p.n2e_dsp, ow);
         c->d_n2e_le = dupload(p.n2e_le, ow);
         c->d_if_l = dupload(p.if_l, ow);
@@ -1070,12 +1334,12 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         const int per_cu = std::max(1, (int)((160 * 1024) / std::max<uint32_t>(c->lds_conv, 1)));
         c->grid = o->grid > 0 ? o->grid : ncu * std::min(per_cu, 2048 / c->block);
         c->lds_compat = 0;
-        c->ca = ConvArgs{c->d_sub, c->d_e2n, c->d_geo32, c->d_ereg, c->d_rscale, c->d_phi,
+        c->ca = ConvArgs{c->d_sub, c->d_e2n, c->d_geo32, c->d_hgt, c->d_ereg, c->d_rscale, c->d_phi,
                          c->d_dirty, c->d_chg, c->d_act, c->d_wake, nullptr, nullptr, nullptr,
                          nullptr, nullptr, c->d_stats, c->d_cnt + 3, c->d_n2e_dsp, c->d_n2e_le,
                          c->nsub, c->ncopy, (float)(o->eps_rel > 0 ? o->eps_rel : 1e-6),
                          (float)o->window, o->local_iters, o_phin, o_flg, o_tch, o_dfr, o_e2n,
-                         o_geo, o_list, o_vlist, o_misc};
+                         o_geo, o_list, o_vlist, o_misc, o_wrt};
         c->sa = SyncArgs{c->d_sub, c->d_if_l, c->d_if_dsp, c->d_if_other, c->d_c2s, c->d_phi,
                          c->d_dirty, c->d_chg, c->d_act, c->d_wake, nullptr, nullptr, nullptr,
                          nullptr, c->d_cnt + 3, c->nsub, c->ncopy};
