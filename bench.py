@@ -65,7 +65,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--max-nn", type=int, default=2048)
+    ap.add_argument("--max-nn", type=int, default=1024)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--plain-steps", type=int, default=3)

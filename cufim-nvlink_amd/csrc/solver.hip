@@ -77,6 +77,8 @@ struct ConvArgs {
   uint32_t *zero_count;  // counter of the list after next: zeroed here.  It
                          // was consumed by the previous iteration and is
                          // appended to only from the next iteration on.
+  uint32_t *grab;        // work-item counter of the current list
+  uint32_t *zero_grab;   // work-item counter of the next list, zeroed here
   unsigned long long *stats;  // activations, sweeps, element evaluations,
                               // iterations, algorithmic bytes
   const uint32_t *iter;  // global iteration counter (advanced by the sync kernel)
@@ -116,6 +118,7 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
   const uint32_t n_work = *a.count;
   if (blockIdx.x == 0 && tid == 0) {
     *a.zero_count = 0;
+    *a.zero_grab = 0;
     if (n_work > 0) atomicAdd(&a.stats[3], 1ull);
   }
   unsigned long long sweeps = 0, evals = 0, acts = 0, abytes = 0, vupd = 0;
@@ -441,12 +444,24 @@ __global__ __launch_bounds__(BS) void k_conv_band(ConvArgs a) {
   const uint32_t n_work = *a.count;
   if (blockIdx.x == 0 && tid == 0) {
     *a.zero_count = 0;
+    *a.zero_grab = 0;
     if (n_work > 0) atomicAdd(&a.stats[3], 1ull);
   }
   unsigned long long sweeps = 0, evals = 0, acts = 0, abytes = 0, vupd = 0;
   const float W = a.window > 0.0f ? a.window * (float)(*a.iter + 1u) : F_INF;
 
-  for (uint32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
+  // the first item of each workgroup is its own index; later items are
+  // taken one at a time from the list's counter, so workgroups that drew
+  // cheap items take more of them (no atomic traffic while the list is
+  // shorter than the grid)
+  auto next_item = [&]() -> uint32_t {
+    if (tid == 0) misc[4] = gridDim.x + atomicAdd(a.grab, 1u);
+    __syncthreads();
+    const uint32_t nw = misc[4];
+    __syncthreads();
+    return nw;
+  };
+  for (uint32_t w = blockIdx.x; w < n_work; w = next_item()) {
     const uint32_t entry = a.list[w];
     const uint32_t inst = entry / a.nsub, s = entry - inst * a.nsub;
     const SubInfo si = a.sub[s];
@@ -1165,6 +1180,8 @@ void enqueue_iteration(fim_ctx *c, int k, bool timed) {
   a.next_list = c->d_list[q];
   a.next_count = c->d_cnt + q;
   a.zero_count = c->d_cnt + z;
+  a.grab = c->d_cnt + 4 + p;
+  a.zero_grab = c->d_cnt + 4 + q;
   launch_conv_compute(c, a);
   if (timed) HIPCHECK(hipEventRecord(c->kev[2 * k + 1], c->stream));
   SyncArgs s = c->sa;
@@ -1301,7 +1318,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         const uint32_t o_list = o_geo + (o->staged ? al16(24 * (uint32_t)c->ne_max) : al16(4 * ne32));
         const uint32_t o_vlist = o_list + al16(2 * (uint32_t)c->ne_max);
         const uint32_t o_misc = o_vlist + (o->staged ? 0u : al16(2 * nn64));
-        const uint32_t o_wrt = o_misc + 16;
+        const uint32_t o_wrt = o_misc + 32;  // misc: 8 words
         c->lds_conv = o_wrt + (o->staged ? 0u : al16(4 * (nn64 / 32)));
         if (c->lds_conv > 160 * 1024)
           throw Error(FIM_ELIMIT, "largest submesh needs " + std::to_string(c->lds_conv) +
@@ -1325,7 +1342,8 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         c->d_act = dalloc<uint32_t>(nss, ow);
         c->d_wake = dalloc<float>(nss, ow);
         for (int k = 0; k < 3; ++k) c->d_list[k] = dalloc<uint32_t>(nss, ow);
-        c->d_cnt = dalloc<uint32_t>(4, ow);  // three list counters + iteration
+        // three list counters, the iteration counter, three work-item counters
+        c->d_cnt = dalloc<uint32_t>(8, ow);
         c->d_rscale = dalloc<float>(4 * (size_t)c->ninst, ow);
         c->d_tmpf = dalloc<float>(c->nn, ow);
         k_fill<float><<<blocks_for(4 * (size_t)c->ninst, 256), 256, 0, c->stream>>>(c->d_rscale, 4 * (size_t)c->ninst, 1.0f);
@@ -1336,7 +1354,8 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         c->lds_compat = 0;
         c->ca = ConvArgs{c->d_sub, c->d_e2n, c->d_geo32, c->d_hgt, c->d_ereg, c->d_rscale, c->d_phi,
                          c->d_dirty, c->d_chg, c->d_act, c->d_wake, nullptr, nullptr, nullptr,
-                         nullptr, nullptr, c->d_stats, c->d_cnt + 3, c->d_n2e_dsp, c->d_n2e_le,
+                         nullptr, nullptr, nullptr, nullptr, c->d_stats, c->d_cnt + 3,
+                         c->d_n2e_dsp, c->d_n2e_le,
                          c->nsub, c->ncopy, (float)(o->eps_rel > 0 ? o->eps_rel : 1e-6),
                          (float)o->window, o->local_iters, o_phin, o_flg, o_tch, o_dfr, o_e2n,
                          o_geo, o_list, o_vlist, o_misc, o_wrt};
@@ -1400,7 +1419,7 @@ int fim_reset(fim_ctx *c) {
       k_fill<uint8_t><<<blocks_for(ncs, 256), 256, 0, c->stream>>>(c->d_chg, ncs, (uint8_t)0);
       k_fill<uint32_t><<<blocks_for(nss, 256), 256, 0, c->stream>>>(c->d_act, nss, 0u);
       k_fill<float><<<blocks_for(nss, 256), 256, 0, c->stream>>>(c->d_wake, nss, F_INF);
-      k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_cnt, 4, 0u);
+      k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_cnt, 8, 0u);
     } else {
       k_fill<double><<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_phi64, c->ncopy, DBL_MAX);
       k_fill<int8_t><<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_st, c->ncopy, (int8_t)FIM_FAR);
