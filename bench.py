@@ -50,13 +50,20 @@ def solve_batch(s, sets):
     return s.run(max_iters=1_000_000)
 
 
-def traffic_from_profiles():
-    """HBM bytes per compute launch from the committed PMC summary."""
-    p = os.path.join(ROOT, "profiles", "conv_compute_slab_pmc.md")
+def traffic_from_profiles(args):
+    """HBM bytes per compute launch from the committed PMC summary, when this
+    run's configuration is the one that was profiled (else None)."""
+    p = os.path.join(ROOT, "profiles", "conv_band_slab_pmc.md")
     if not os.path.exists(p):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read())
-    return float(m.group(1)) if m else None
+    text = open(p).read()
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    c = re.search(r"traffic_config:\s*(.*)", text)
+    mine = (f"batch={args.batch} max_nn={args.max_nn} block={args.block} "
+            f"window={float(args.window)} staged={args.staged}")
+    if not m or not c or c.group(1).strip() != mine:
+        return None
+    return float(m.group(1))
 
 
 def main():
@@ -86,7 +93,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one MI355X only: the batched solve would shard by map across "
-              "GPUs with no exchange, but no multi-GPU run is part of this build.", file=sys.stderr)
+              "GPUs with no exchange; only the single-GPU measurement is implemented.", file=sys.stderr)
         sys.exit(1)
 
     cfg = W.SLAB
@@ -168,7 +175,7 @@ def main():
     cpu = None
     if not (args.no_check and args.no_cpu_baseline):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as O  # checker / CPU baseline only
+        import pyoracle as O  # accuracy check and CPU baseline only
         xyz, e2n, _ = mesh.arrays()
         fib, vel = mesh.material()
         om = O.Mesh(xyz, e2n, fib=fib, vel=vel)
@@ -234,7 +241,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(),
+            "traffic": traffic_from_profiles(args),
             "alg_bytes_per_launch": bytes_per_launch,
             "avg_launch_us": 1e6 * avg_launch_s,
             "launches": kern["compute_launches"],

@@ -1,5 +1,6 @@
 // host_capi.cpp -- C ABI of the host-side objects (fim_mesh, fim_pack) and
 // the thread-local error message.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -157,6 +158,54 @@ int fim_mesh_gen_box(double lx, double ly, double lz, int nx, int ny, int nz, do
       throw;
     }
     *out = m;
+  });
+}
+
+int fim_mesh_concat(const fim_mesh *const *parts, int32_t n, fim_mesh **out) {
+  return guarded([&] {
+    if (!parts || n <= 0 || !out) throw Error(FIM_EINVAL, "bad concat arguments");
+    *out = nullptr;
+    fim_mesh *u = new fim_mesh();
+    try {
+      Mesh &m = u->m;
+      bool any_region = false, all_tags = true;
+      for (int32_t i = 0; i < n; ++i) {
+        if (!parts[i]) throw Error(FIM_EINVAL, "null mesh in concat");
+        any_region |= !parts[i]->m.region.empty();
+        all_tags &= !parts[i]->m.atags.empty();
+      }
+      int32_t tag_base = 0;
+      for (int32_t i = 0; i < n; ++i) {
+        const Mesh &p = parts[i]->m;
+        const int64_t vo = m.nn;
+        m.xyz.insert(m.xyz.end(), p.xyz.begin(), p.xyz.end());
+        for (int64_t v : p.e2n) m.e2n.push_back(v + vo);
+        m.fib.insert(m.fib.end(), p.fib.begin(), p.fib.end());
+        m.vel.insert(m.vel.end(), p.vel.begin(), p.vel.end());
+        m.etags.insert(m.etags.end(), p.etags.begin(), p.etags.end());
+        if (any_region) {
+          if (p.region.empty())
+            m.region.insert(m.region.end(), p.ne, 0);
+          else
+            m.region.insert(m.region.end(), p.region.begin(), p.region.end());
+        }
+        if (all_tags) {  // keep the parts' partitions apart
+          int32_t mx = 0;
+          for (int32_t t : p.atags) {
+            m.atags.push_back(t + tag_base);
+            mx = std::max(mx, t);
+          }
+          tag_base += mx + 1;
+        }
+        m.nn += p.nn;
+        m.ne += p.ne;
+      }
+      m.finalize();
+    } catch (...) {
+      delete u;
+      throw;
+    }
+    *out = u;
   });
 }
 

@@ -90,16 +90,45 @@ Partition partition_rcb(const Mesh &m, int max_nn, int max_ne, uint64_t seed) {
     }
   std::vector<uint64_t> key(m.ne);
   for (int64_t e = 0; e < m.ne; ++e) key[e] = mix64(seed ^ mix64((uint64_t)e + 1));
+  // connected components (elements sharing a vertex), numbered by their
+  // lowest element: bisection never mixes components, so a union of meshes
+  // partitions as its parts would on their own
+  std::vector<int64_t> uf(m.nn);
+  std::iota(uf.begin(), uf.end(), 0);
+  auto find = [&](int64_t v) {
+    while (uf[v] != v) v = uf[v] = uf[uf[v]];
+    return v;
+  };
+  for (int64_t e = 0; e < m.ne; ++e)
+    for (int k = 1; k < 4; ++k) {
+      const int64_t a = find(m.e2n[4 * e]), b = find(m.e2n[4 * e + k]);
+      if (a != b) uf[std::max(a, b)] = std::min(a, b);
+    }
+  std::vector<int64_t> comp_of_root(m.nn, -1), ecomp(m.ne);
+  int64_t ncomp = 0;
+  for (int64_t e = 0; e < m.ne; ++e) {
+    const int64_t r = find(m.e2n[4 * e]);
+    if (comp_of_root[r] < 0) comp_of_root[r] = ncomp++;
+    ecomp[e] = comp_of_root[r];
+  }
   std::vector<int64_t> order(m.ne);
   std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t x, int64_t y) { return ecomp[x] < ecomp[y]; });
   std::vector<int64_t> stamp(m.nn, -1);
   int64_t stamp_id = 0;
   Partition p;
   p.e2s.assign(m.ne, -1);
   p.nsub = 0;
-  // explicit stack of [begin, end) ranges of `order`
+  // explicit stack of [begin, end) ranges of `order`, one per component to
+  // start, first component on top
   std::vector<std::pair<int64_t, int64_t>> stack;
-  stack.push_back({0, m.ne});
+  for (int64_t i = m.ne; i > 0;) {
+    int64_t j = i - 1;
+    while (j > 0 && ecomp[order[j - 1]] == ecomp[order[i - 1]]) --j;
+    stack.push_back({j, i});
+    i = j;
+  }
   while (!stack.empty()) {
     auto [b, e] = stack.back();
     stack.pop_back();

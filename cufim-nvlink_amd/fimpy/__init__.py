@@ -87,6 +87,7 @@ SIGNATURES = {
     "fim_mesh_gen_box": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                    C.c_double, C.c_uint64, C.c_double, C.c_double, C.c_double,
                                    C.c_double, C.POINTER(_vp)]),
+    "fim_mesh_concat": (C.c_int, [C.POINTER(_vp), C.c_int32, C.POINTER(_vp)]),
     "fim_pack_build": (C.c_int, [_vp, C.POINTER(PartOpts), C.POINTER(_vp)]),
     "fim_pack_info": (C.c_int, [_vp, C.POINTER(PackInfo)]),
     "fim_pack_partition": (C.c_int, [_vp, _pi64, _pi64]),
@@ -167,6 +168,13 @@ class Mesh:
         h = _vp()
         _check(lib().fim_mesh_gen_box(lx, ly, lz, nx, ny, nz, jitter, seed, angle0, angle1, vf, vc,
                                       C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def concat(cls, parts):
+        arr = (_vp * len(parts))(*[q.h for q in parts])
+        h = _vp()
+        _check(lib().fim_mesh_concat(arr, len(parts), C.byref(h)))
         return cls(h)
 
     def read_tags(self, path):

@@ -1,0 +1,107 @@
+"""Autotuner of the converge-mode solver: picks the submesh size, the
+workgroup shape, the local sweeps per global iteration and the time window
+for a mesh and batch size by measuring short batched solves on the GPU.
+
+Search: coordinate descent from the defaults over
+    max_nn      {256, 512, 1024, 2048}   vertex copies per submesh
+    block       {64, 128, 256}           workgroup size
+    local_iters {0, 1, 2, 4, 8}          sweeps per activation (0 = to local convergence)
+    window      {500, 1000, 2000, 4000}  microseconds per global iteration
+Each trial times one batch solve after one warm-up solve.  Results are cached
+in a JSON file keyed by a mesh signature (sizes, bounding box, coordinate
+checksum) and the batch size.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import time
+
+import numpy as np
+
+from . import PART_RCB, Mesh, Pack, Solver
+
+SPACE = {
+    "max_nn": [256, 512, 1024, 2048],
+    "block": [64, 128, 256],
+    "local_iters": [0, 1, 2, 4, 8],
+    "window": [500.0, 1000.0, 2000.0, 4000.0],
+}
+DEFAULT = {"max_nn": 1024, "block": 256, "local_iters": 0, "window": 2000.0}
+
+
+def signature(mesh: Mesh, n_instances: int) -> str:
+    xyz, e2n, _ = mesh.arrays()
+    h = hashlib.sha256()
+    h.update(np.array([mesh.nn, mesh.ne, n_instances], dtype=np.int64).tobytes())
+    h.update(np.round(xyz.min(axis=0), 3).tobytes())
+    h.update(np.round(xyz.max(axis=0), 3).tobytes())
+    h.update(np.float64(np.round(xyz.sum(), 1)).tobytes())
+    h.update(np.int64(e2n.sum()).tobytes())
+    return h.hexdigest()[:24]
+
+
+def _measure(mesh, packs, cfg, seeds, scales, seed):
+    key = cfg["max_nn"]
+    if key not in packs:
+        packs[key] = Pack(mesh, PART_RCB, max_nn=key, seed=seed)
+    s = Solver(packs[key], n_instances=len(seeds), block=cfg["block"],
+               local_iters=cfg["local_iters"], window=cfg["window"])
+    try:
+        def solve():
+            s.reset()
+            for i, src in enumerate(seeds):
+                s.seed(src, 0.0, inst=i)
+                if scales is not None:
+                    s.set_scaling(scales[i], inst=i)
+            s.run(max_iters=1_000_000)
+        solve()
+        t0 = time.perf_counter()
+        solve()
+        dt = time.perf_counter() - t0
+    finally:
+        s.close()
+    return len(seeds) / dt
+
+
+def tune(mesh: Mesh, seeds, scales=None, space=None, cache=None, seed=0, rounds=2, log=None):
+    """Returns (best config, list of (config, maps/s) trials).  `seeds` is a
+    list of per-instance pacing vertex arrays; `scales` optional (n, 4)."""
+    space = space or SPACE
+    sig = signature(mesh, len(seeds))
+    if cache and os.path.exists(cache):
+        db = json.load(open(cache))
+        if sig in db:
+            return db[sig]["best"], db[sig]["trials"]
+    packs = {}
+    trials = []
+    seen = {}
+
+    def run(cfg):
+        k = json.dumps(cfg, sort_keys=True)
+        if k not in seen:
+            seen[k] = _measure(mesh, packs, cfg, seeds, scales, seed)
+            trials.append((dict(cfg), seen[k]))
+            if log:
+                log(f"[autotune] {cfg} -> {seen[k]:.1f} maps/s")
+        return seen[k]
+
+    best = {k: (DEFAULT[k] if DEFAULT[k] in v else v[0]) for k, v in space.items()}
+    best_v = run(best)
+    for _ in range(rounds):
+        improved = False
+        for k in ("window", "max_nn", "block", "local_iters"):
+            for v in space[k]:
+                cand = dict(best, **{k: v})
+                r = run(cand)
+                if r > best_v:
+                    best, best_v, improved = cand, r, True
+        if not improved:
+            break
+    if cache:
+        db = json.load(open(cache)) if os.path.exists(cache) else {}
+        db[sig] = {"best": best, "maps_per_s": best_v, "trials": trials}
+        os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+        json.dump(db, open(cache, "w"), indent=1)
+    return best, trials

@@ -62,3 +62,53 @@ def pacing_sites(cfg, k, seed):
     """k distinct vertices of the z = 0 face, from the seed."""
     face = face_vertices(cfg, 0)
     return face[pick(face.shape[0], k, seed)]
+
+
+# ---------------------------------------------------------------------------
+# batched workload over four meshes of uneven size, built as the cube and the
+# slab above: 48k, ~400k, ~1M and ~2M tets (the slabs keep the 625 um
+# in-plane spacing and shrink the square)
+# ---------------------------------------------------------------------------
+MULTI = [
+    ("cube_48k", CUBE),
+    ("slab_404k", dict(SLAB, nx=72, ny=72, lx=45000.0, ly=45000.0)),
+    ("slab_996k", dict(SLAB, nx=113, ny=113, lx=70625.0, ly=70625.0)),
+    ("slab_2M", SLAB),
+]
+
+
+def quadrant_regions(mesh: Mesh) -> np.ndarray:
+    """Region id 0..3 of each element: quadrant of its centroid in the xy
+    plane around the mesh's bounding-box centre."""
+    xyz, e2n, _ = mesh.arrays()
+    c = xyz[e2n].mean(axis=1)
+    mid = 0.5 * (xyz.min(axis=0) + xyz.max(axis=0))
+    return ((c[:, 0] > mid[0]).astype(np.uint8) + 2 * (c[:, 1] > mid[1]).astype(np.uint8))
+
+
+def multi_mesh(parts, seed):
+    """Union of box meshes with quadrant regions.  Returns the union mesh,
+    the vertex offset of each part and the parts' configs."""
+    meshes, offs, off = [], [], 0
+    for k, (_, cfg) in enumerate(parts):
+        m = box(cfg, seed + k)
+        m.set_regions(quadrant_regions(m))
+        meshes.append(m)
+        offs.append(off)
+        off += m.nn
+    return Mesh.concat(meshes), offs, [cfg for _, cfg in parts]
+
+
+def multi_solves(parts, offs, n, seed):
+    """n solves spread over the parts in turn: 1-3 pacing vertices on the
+    part's z = 0 face and four regional speed factors in [0.5, 1.5]."""
+    g = splitmix64(seed)
+    out = []
+    for i in range(n):
+        k = i % len(parts)
+        cfg = parts[k]
+        npace = 1 + next(g) % 3
+        sites = pacing_sites(cfg, npace, next(g)) + offs[k]
+        scale = np.array([0.5 + (next(g) >> 11) * 2.0 ** -53 for _ in range(4)], dtype=np.float32)
+        out.append(dict(part=k, sites=sites, scale=scale))
+    return out

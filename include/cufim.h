@@ -116,6 +116,12 @@ int fim_mesh_gen_box(double lx, double ly, double lz, int nx, int ny, int nz,
                      double jitter, uint64_t seed, double angle0, double angle1,
                      double vf, double vc, fim_mesh **out);
 
+/* Disjoint union of meshes (vertices and elements renumbered by offsets, in
+ * argument order; tags, fibres, speeds and regions carried over).  One
+ * context on the union batches solves of different meshes: an instance
+ * seeded in one component never activates the others. */
+int fim_mesh_concat(const fim_mesh *const *parts, int32_t n, fim_mesh **out);
+
 /* ========================================================================
  * Pack: the submesh decomposition in device layout (replaces
  * extract_submeshes2 + smsh_t/eksv_t/inter3_t blobs, src/core/types.cpp:

@@ -1,5 +1,5 @@
 /*
- * fim_oracle.c -- CPU restatement of the reference FIM solve (checker only).
+ * fim_oracle.c -- CPU restatement of the reference FIM solve (test oracle only).
  *
  * TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg load this library; the product never does.

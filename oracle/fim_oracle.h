@@ -1,7 +1,7 @@
 /*
  * fim_oracle.h -- CPU restatement of the reference FIM activation-time solve.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP solver:
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity oracle of the HIP solver:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.  The product (cufim-nvlink_amd/) never links, loads or calls it.
  *
@@ -19,8 +19,9 @@
  *                     same discrete local update (plain FIM to convergence);
  *                     it pins the converged maps of the GPU solver.
  *
- * Pinning status of each function is stated at its definition in
- * fim_oracle.c and summarised in DESIGN.md.
+ * Pinning status of each function (which golden vector, reference build,
+ * closed form or brute force checks it) is stated at its definition in
+ * fim_oracle.c.
  */
 #ifndef FIM_ORACLE_H_
 #define FIM_ORACLE_H_

@@ -11,7 +11,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 UT = os.path.join(GOLDEN, "ut_mesh")
 
 
-from fimpy.workloads import CUBE as C1, SLAB as C2, pacing_sites, pick, scaled, splitmix64  # noqa: F401,E402
+from fimpy.workloads import CUBE, SLAB, pacing_sites, pick, scaled, splitmix64  # noqa: F401,E402
 
 
 def box(cfg, seed):

@@ -71,7 +71,7 @@ def test_compat_ut_mesh_windows(steps):
 
 def test_compat_box_rcb_partition():
     """reference semantics on a generated mesh with the own partitioner"""
-    cfg = cases.scaled(cases.C1, 2)
+    cfg = cases.scaled(cases.CUBE, 2)
     m, om = cases.box(cfg, seed=3)
     p = F.Pack(m, F.PART_RCB, max_nn=128, seed=3)
     e2s, _ = p.partition()
@@ -105,8 +105,8 @@ def _check_fp32(phi, ref):
     return err
 
 
-def test_converge_config1_vs_fp64_oracle():
-    m, om = cases.box(cases.C1, seed=1)
+def test_converge_cube_48k_vs_fp64_oracle():
+    m, om = cases.box(cases.CUBE, seed=1)
     p = F.Pack(m, F.PART_RCB, max_nn=512, seed=1)
     phi, st, it, res, stats = _converge(m, p, [0])
     ref, _ = O.fim_solve(om, [0], variant=O.LOCAL_EXACT)
@@ -116,7 +116,7 @@ def test_converge_config1_vs_fp64_oracle():
 
 
 def test_converge_anisotropic_slab_vs_fp64_oracle():
-    cfg = cases.scaled(cases.C2, 4)
+    cfg = cases.scaled(cases.SLAB, 4)
     m, om = cases.box(cfg, seed=2)
     p = F.Pack(m, F.PART_RCB, max_nn=512, seed=2)
     src = cases.pacing_sites(cfg, 5, seed=2)
@@ -128,7 +128,7 @@ def test_converge_anisotropic_slab_vs_fp64_oracle():
 @pytest.mark.parametrize("staged", [False, True])
 @pytest.mark.parametrize("block", [64, 128, 256, 512])
 def test_converge_workgroup_shapes_agree(block, staged):
-    cfg = cases.scaled(cases.C2, 8)
+    cfg = cases.scaled(cases.SLAB, 8)
     m, om = cases.box(cfg, seed=5)
     p = F.Pack(m, F.PART_RCB, max_nn=256, seed=5)
     src = cases.pacing_sites(cfg, 3, seed=5)
@@ -138,7 +138,7 @@ def test_converge_workgroup_shapes_agree(block, staged):
 
 
 def test_converge_deterministic_and_batched():
-    cfg = cases.scaled(cases.C2, 8)
+    cfg = cases.scaled(cases.SLAB, 8)
     m, om = cases.box(cfg, seed=7)
     p = F.Pack(m, F.PART_RCB, max_nn=256, seed=7)
     srcs = [cases.pacing_sites(cfg, k + 1, seed=100 + k) for k in range(4)]
@@ -159,7 +159,7 @@ def test_converge_deterministic_and_batched():
 @pytest.mark.parametrize("window", [300.0, 1000.0, 4000.0])
 def test_converge_time_window_same_fixed_point(window, staged):
     """the per-iteration time window changes the processing order only"""
-    cfg = cases.scaled(cases.C2, 4)
+    cfg = cases.scaled(cases.SLAB, 4)
     m, om = cases.box(cfg, seed=2)
     p = F.Pack(m, F.PART_RCB, max_nn=512, seed=2)
     src = cases.pacing_sites(cfg, 5, seed=2)
@@ -172,7 +172,7 @@ def test_converge_time_window_same_fixed_point(window, staged):
 def test_warm_start_from_upper_bound_reaches_same_fixed_point():
     """init_field with every vertex active: an upper bound (oracle + 5 %)
     converges to the cold-start fixed point"""
-    cfg = cases.scaled(cases.C2, 8)
+    cfg = cases.scaled(cases.SLAB, 8)
     m, om = cases.box(cfg, seed=9)
     p = F.Pack(m, F.PART_RCB, max_nn=256, seed=9)
     src = cases.pacing_sites(cfg, 2, seed=9)
@@ -192,7 +192,7 @@ def test_warm_start_from_upper_bound_reaches_same_fixed_point():
 def test_regional_scaling_equals_scaled_mesh():
     """one instance with all four regional factors s equals the mesh with
     both speeds multiplied by s (times divide by s)"""
-    cfg = cases.scaled(cases.C1, 4)
+    cfg = cases.scaled(cases.CUBE, 4)
     m, om = cases.box(cfg, seed=4)
     p = F.Pack(m, F.PART_RCB, max_nn=128, seed=4)
     s = F.Solver(p, n_instances=2)

@@ -131,7 +131,7 @@ def test_plan_pulse_trains_and_errors(tmp_path):
 
 
 def test_rcb_partition_is_deterministic_and_valid():
-    m = F.Mesh.gen_box(seed=3, **cases.scaled(cases.C2, 4))
+    m = F.Mesh.gen_box(seed=3, **cases.scaled(cases.SLAB, 4))
     a = F.Pack(m, F.PART_RCB, max_nn=300, seed=11)
     b = F.Pack(m, F.PART_RCB, max_nn=300, seed=11)
     ea, ca = a.partition()
@@ -150,7 +150,7 @@ def test_rcb_partition_is_deterministic_and_valid():
 
 
 def test_box_generator_is_seeded_and_conforming():
-    cfg = dict(cases.C1, nx=5, ny=4, nz=3)
+    cfg = dict(cases.CUBE, nx=5, ny=4, nz=3)
     a = F.Mesh.gen_box(seed=1, **cfg)
     b = F.Mesh.gen_box(seed=1, **cfg)
     c = F.Mesh.gen_box(seed=2, **cfg)
@@ -183,7 +183,7 @@ def test_box_generator_is_seeded_and_conforming():
 def test_geometry_matches_oracle_metric():
     """product metric geometry (via the pack) agrees with the oracle's
     closed form e^T M e, checked through orc_geo_elem on the same input"""
-    m = F.Mesh.gen_box(seed=5, **dict(cases.C2, nx=4, ny=4, nz=2))
+    m = F.Mesh.gen_box(seed=5, **dict(cases.SLAB, nx=4, ny=4, nz=2))
     xyz, e2n, _ = m.arrays()
     fib, vel = m.material()
     for e in range(0, e2n.shape[0], 5):
@@ -228,3 +228,44 @@ def test_cli_usage_and_arguments():
     r = subprocess.run([cli, "--mesh=/nonexistent.vtk", "--part=x", "--plan=y", "--odir=z"],
                        capture_output=True, text=True, cwd=ROOT)
     assert r.returncode != 0 and "file not found" in r.stderr
+
+
+def test_mesh_concat_is_disjoint_union():
+    from fimpy import workloads as W
+    a = F.Mesh.gen_box(seed=1, **dict(cases.CUBE, nx=3, ny=3, nz=2))
+    b = F.Mesh.gen_box(seed=2, **dict(cases.SLAB, nx=4, ny=3, nz=2))
+    a.set_regions(W.quadrant_regions(a))
+    b.set_regions(W.quadrant_regions(b))
+    u = F.Mesh.concat([a, b])
+    assert u.nn == a.nn + b.nn and u.ne == a.ne + b.ne
+    xa, ea, _ = a.arrays()
+    xb, eb, _ = b.arrays()
+    xu, eu, _ = u.arrays()
+    assert np.array_equal(xu, np.vstack([xa, xb]))
+    assert np.array_equal(eu, np.vstack([ea, eb + a.nn]))
+    fa, va = a.material()
+    fb, vb = b.material()
+    fu, vu = u.material()
+    assert np.array_equal(fu, np.vstack([fa, fb])) and np.array_equal(vu, np.vstack([va, vb]))
+    # a partition of the union never mixes components
+    p = F.Pack(u, F.PART_RCB, max_nn=40, seed=1)
+    e2s, _ = p.partition()
+    assert not set(e2s[:a.ne]) & set(e2s[a.ne:])
+    with pytest.raises(F.FimError, match="FIM_EINVAL"):
+        F.Mesh.concat([])
+
+
+def test_multi_mesh_workload_is_seeded():
+    from fimpy import workloads as W
+    parts = [(n, W.scaled(c, 8)) for n, c in W.MULTI]
+    m, offs, cfgs = W.multi_mesh(parts, 1)
+    s1 = W.multi_solves(cfgs, offs, 12, 3)
+    s2 = W.multi_solves(cfgs, offs, 12, 3)
+    for x, y in zip(s1, s2):
+        assert x["part"] == y["part"] and np.array_equal(x["sites"], y["sites"])
+        assert np.array_equal(x["scale"], y["scale"])
+    bounds = offs + [m.nn]
+    for s in s1:
+        k = s["part"]
+        assert np.all((s["sites"] >= bounds[k]) & (s["sites"] < bounds[k + 1]))
+        assert np.all((s["scale"] >= 0.5) & (s["scale"] <= 1.5))

@@ -320,7 +320,7 @@ def test_reference_tet_formula_rarely_accepts():
 def test_fixed_point_bounds_and_residual():
     """FP64 fixed point: zero residual; above the continuous lower bound
     (metric distance to the source); below the edge-graph shortest path"""
-    cfg = cases.scaled(cases.C1, 2)
+    cfg = cases.scaled(cases.CUBE, 2)
     import fimpy as F  # generator only (host code, no GPU)
     m = F.Mesh.gen_box(seed=11, **cfg)
     xyz, e2n, _ = m.arrays()
@@ -354,7 +354,7 @@ def test_fixed_point_bounds_and_residual():
 def test_fixed_point_is_order_independent():
     """the label-correcting result equals plain Jacobi sweeps to a fixed
     point (tiny mesh)"""
-    cfg = dict(cases.C1, nx=4, ny=4, nz=4)
+    cfg = dict(cases.CUBE, nx=4, ny=4, nz=4)
     import fimpy as F
     m = F.Mesh.gen_box(seed=12, **cfg)
     xyz, e2n, _ = m.arrays()
