@@ -61,9 +61,20 @@ def traffic_from_profiles(args):
     c = re.search(r"traffic_config:\s*(.*)", text)
     mine = (f"batch={args.batch} max_nn={args.max_nn} block={args.block} "
             f"window={float(args.window)} staged={args.staged}")
-    if not m or not c or c.group(1).strip() != mine:
+    src = re.search(r"traffic_source:\s*([0-9a-f]+)", text)
+    if not m or not c or c.group(1).strip() != mine or not src or src.group(1) != kernel_source_digest():
         return None
     return float(m.group(1))
+
+
+def kernel_source_digest():
+    """sha256 (first 16 hex digits) of the kernel sources; a PMC summary
+    applies only to the sources it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("solver.hip", "local_solve.hpp"):
+        h.update(open(os.path.join(ROOT, "cufim-nvlink_amd", "csrc", f), "rb").read())
+    return h.hexdigest()[:16]
 
 
 def main():

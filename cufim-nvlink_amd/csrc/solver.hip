@@ -82,8 +82,11 @@ struct ConvArgs {
   unsigned long long *stats;  // activations, sweeps, element evaluations,
                               // iterations, algorithmic bytes
   const uint32_t *iter;  // global iteration counter (advanced by the sync kernel)
-  const uint32_t *n2e_dsp;  // per copy: incident local elements (unstaged kernel)
+  const uint32_t *n2e_dsp;  // per copy: incident local elements (staged kernel)
   const uint16_t *n2e_le;
+  const uint4 *erec;        // band kernel: 48 B per element (ElemRecord layout)
+  const uint32_t *adj_ptr;  // band kernel: per copy, chunk offset << 6 | chunk count
+  const uint4 *adj;         // 8 incident local element ids (u16) per chunk, 0xffff padded
   uint32_t nsub, ncopy;
   float eps;
   float window;          // per-iteration time window, 0 = off
@@ -410,6 +413,27 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
   if (lane == 0 && vupd) atomicAdd(&a.stats[5], vupd);
 }
 
+// Element record of the band kernel, three 16-byte words per element:
+//   [0] nodes 0,1 (u16 each; bit 15 and bit 31 carry the region id's bits 0
+//       and 1), nodes 2,3 (u16 each), geometry e12 e13 (f32)
+//   [1] geometry e23 e14 e24 e34 (f32)
+//   [2] node heights h0..h3 (f32)
+// Local copy ids are below 2^15, so the two region bits are free.
+constexpr uint32_t kAdjPad = 0xffffffffu;
+
+__device__ inline void mark_word(uint32_t *emark, uint32_t w) {
+  const uint32_t lo = w & 0xffffu, hi = w >> 16;
+  if (lo != 0xffffu) atomicOr(&emark[lo >> 5], 1u << (lo & 31));
+  if (hi != 0xffffu) atomicOr(&emark[hi >> 5], 1u << (hi & 31));
+}
+
+__device__ inline void mark_chunk(uint32_t *emark, const uint4 q) {
+  mark_word(emark, q.x);
+  mark_word(emark, q.y);
+  mark_word(emark, q.z);
+  mark_word(emark, q.w);
+}
+
 // expands the set bits of `bits` (word k of a bitmask) into list[base...]
 __device__ inline void expand_bits(uint32_t bits, uint32_t k, uint16_t *list, uint32_t base) {
   while (bits) {
@@ -528,14 +552,23 @@ __global__ __launch_bounds__(BS) void k_conv_band(ConvArgs a) {
         break;
       }
       ++it;
-      // mark the elements incident to the sources
+      // mark the elements incident to the sources: up to four 8-entry chunks
+      // of the padded adjacency are loaded before any is used
       for (uint32_t i = tid; i < nsrc; i += BS) {
         const uint32_t c = si.coff + vlist[i];
-        const uint32_t kb = a.n2e_dsp[c], ke = a.n2e_dsp[c + 1];
-        for (uint32_t k = kb; k < ke; ++k) {
-          const uint32_t le = a.n2e_le[k];
-          atomicOr(&emark[le >> 5], 1u << (le & 31));
-        }
+        const uint32_t ap = a.adj_ptr[c];
+        const uint4 *ch = a.adj + (ap >> 6);
+        const uint32_t nch = ap & 63u;
+        const uint4 pad = make_uint4(kAdjPad, kAdjPad, kAdjPad, kAdjPad);
+        const uint4 q0 = ch[0];
+        const uint4 q1 = nch > 1 ? ch[1] : pad;
+        const uint4 q2 = nch > 2 ? ch[2] : pad;
+        const uint4 q3 = nch > 3 ? ch[3] : pad;
+        mark_chunk(emark, q0);
+        mark_chunk(emark, q1);
+        mark_chunk(emark, q2);
+        mark_chunk(emark, q3);
+        for (uint32_t k = 4; k < nch; ++k) mark_chunk(emark, ch[k]);
       }
       if (tid == 0) misc[0] = 0u;
       __syncthreads();
@@ -554,16 +587,17 @@ __global__ __launch_bounds__(BS) void k_conv_band(ConvArgs a) {
       // candidates of the marked elements
       for (uint32_t i = tid; i < nact; i += BS) {
         const uint32_t e = elist[i];
-        const size_t ge = (size_t)si.eoff + e;
-        const uint2 q = a.e2n[ge];
-        const float sc = rs[a.ereg[ge] & 3];
-        const float2 *g2 = reinterpret_cast<const float2 *>(a.geo + 6 * ge);
-        const float2 x0 = g2[0], x1 = g2[1], x2 = g2[2];
-        const float g[6] = {x0.x * sc, x0.y * sc, x1.x * sc, x1.y * sc, x2.x * sc, x2.y * sc};
+        const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
+        const uint4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+        const float sc = rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
+        const float g[6] = {__uint_as_float(r0.z) * sc, __uint_as_float(r0.w) * sc,
+                            __uint_as_float(r1.x) * sc, __uint_as_float(r1.y) * sc,
+                            __uint_as_float(r1.z) * sc, __uint_as_float(r1.w) * sc};
         const float hs = csqrt(sc);
-        const float4 hh = a.hgt[ge];
-        const float h[4] = {hh.x * hs, hh.y * hs, hh.z * hs, hh.w * hs};
-        const uint32_t nd[4] = {q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16};
+        const float h[4] = {__uint_as_float(r2.x) * hs, __uint_as_float(r2.y) * hs,
+                            __uint_as_float(r2.z) * hs, __uint_as_float(r2.w) * hs};
+        const uint32_t nd[4] = {r0.x & 0x7fffu, (r0.x >> 16) & 0x7fffu, r0.y & 0xffffu,
+                                r0.y >> 16};
         float ph[4];
         int f[4];
 #pragma unroll
@@ -1085,6 +1119,8 @@ struct fim_ctx {
   uint32_t *d_n2e_dsp = nullptr, *d_n2n_dsp = nullptr;
   uint16_t *d_n2e_le = nullptr, *d_n2n_lv = nullptr;
   uint8_t *d_n2e_pos = nullptr;
+  uint4 *d_erec = nullptr, *d_adj = nullptr;
+  uint32_t *d_adj_ptr = nullptr;
   // converge state
   float *d_phi = nullptr, *d_rscale = nullptr;
   uint8_t *d_dirty = nullptr, *d_chg = nullptr;
@@ -1332,6 +1368,38 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         }
         c->d_n2e_dsp = dupload(p.n2e_dsp, ow);
         c->d_n2e_le = dupload(p.n2e_le, ow);
+        if (!o->staged) {
+          if (c->nn_max > 32768) throw Error(FIM_ELIMIT, "submeshes above 32768 copies");
+          std::vector<uint4> rec(3 * (size_t)p.ne_global);
+          for (int64_t e = 0; e < p.ne_global; ++e) {
+            const uint16_t *n = &p.e2n_l[4 * e];
+            const uint32_t r = p.ereg.empty() ? 0u : (p.ereg[e] & 3u);
+            const float *g = &p.geo32[6 * e], *h = &p.hgt32[4 * e];
+            auto fb = [](float x) { uint32_t u; std::memcpy(&u, &x, 4); return u; };
+            rec[3 * e] = make_uint4((uint32_t)n[0] | ((uint32_t)n[1] << 16) | ((r & 1u) << 15) |
+                                        ((r & 2u) << 30),
+                                    (uint32_t)n[2] | ((uint32_t)n[3] << 16), fb(g[0]), fb(g[1]));
+            rec[3 * e + 1] = make_uint4(fb(g[2]), fb(g[3]), fb(g[4]), fb(g[5]));
+            rec[3 * e + 2] = make_uint4(fb(h[0]), fb(h[1]), fb(h[2]), fb(h[3]));
+          }
+          c->d_erec = dupload(rec, ow);
+          std::vector<uint32_t> aptr(p.ncopy);
+          std::vector<uint16_t> adj;
+          for (int64_t cc = 0; cc < p.ncopy; ++cc) {
+            const uint32_t kb = p.n2e_dsp[cc], ke = p.n2e_dsp[cc + 1];
+            const uint32_t nch = (ke - kb + 7) / 8;
+            if (nch == 0 || nch > 63) throw Error(FIM_ELIMIT, "vertex with no or over 504 incident elements");
+            const size_t off = adj.size() / 8;
+            if (off >= (1u << 26)) throw Error(FIM_ELIMIT, "adjacency above 2^26 chunks");
+            aptr[cc] = (uint32_t)(off << 6) | nch;
+            for (uint32_t k = kb; k < ke; ++k) adj.push_back(p.n2e_le[k]);
+            while (adj.size() % 8) adj.push_back(0xffffu);
+          }
+          c->d_adj_ptr = dupload(aptr, ow);
+          std::vector<uint4> adj4(adj.size() / 8);
+          std::memcpy(adj4.data(), adj.data(), adj.size() * 2);
+          c->d_adj = dupload(adj4, ow);
+        }
         c->d_if_l = dupload(p.if_l, ow);
         c->d_if_dsp = dupload(p.if_dsp, ow);
         c->d_if_other = dupload(p.if_other, ow);
@@ -1355,7 +1423,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         c->ca = ConvArgs{c->d_sub, c->d_e2n, c->d_geo32, c->d_hgt, c->d_ereg, c->d_rscale, c->d_phi,
                          c->d_dirty, c->d_chg, c->d_act, c->d_wake, nullptr, nullptr, nullptr,
                          nullptr, nullptr, nullptr, nullptr, c->d_stats, c->d_cnt + 3,
-                         c->d_n2e_dsp, c->d_n2e_le,
+                         c->d_n2e_dsp, c->d_n2e_le, c->d_erec, c->d_adj_ptr, c->d_adj,
                          c->nsub, c->ncopy, (float)(o->eps_rel > 0 ? o->eps_rel : 1e-6),
                          (float)o->window, o->local_iters, o_phin, o_flg, o_tch, o_dfr, o_e2n,
                          o_geo, o_list, o_vlist, o_misc, o_wrt};

@@ -205,3 +205,34 @@ def test_regional_scaling_equals_scaled_mesh():
     s.close()
     fin = a < 1e50
     assert np.allclose(b[fin] * 1.25, a[fin], rtol=2e-6, atol=1e-3)
+
+
+def test_multi_mesh_batch_with_regional_scalings():
+    """one batch over a union of four meshes (cube + three slabs, reduced),
+    each instance paced on one component with four quadrant speed factors;
+    each map equals the oracle's fixed point on the union mesh with every
+    element's speeds multiplied by its region's factor"""
+    from fimpy import workloads as W
+    parts = [(n, cases.scaled(c, 3)) for n, c in W.MULTI]
+    m, offs, cfgs = W.multi_mesh(parts, 7)
+    solves = W.multi_solves(cfgs, offs, 8, 11)
+    p = F.Pack(m, F.PART_RCB, max_nn=512, seed=7)
+    s = F.Solver(p, n_instances=len(solves), window=1000.0)
+    for i, sv in enumerate(solves):
+        s.seed(sv["sites"], 0.0, inst=i)
+        s.set_scaling(sv["scale"], inst=i)
+    s.run()
+    xyz, e2n, _ = m.arrays()
+    fib, vel = m.material()
+    reg = np.concatenate([W.quadrant_regions(F.Mesh.gen_box(seed=7 + k, **c))
+                          for k, (_, c) in enumerate(parts)])
+    bounds = offs + [m.nn]
+    for i, sv in enumerate(solves):
+        phi, _ = s.download(inst=i)
+        om = O.Mesh(xyz, e2n, fib=fib, vel=vel * sv["scale"].astype(np.float64)[reg][:, None])
+        ref, _ = O.fim_solve(om, sv["sites"], variant=O.LOCAL_EXACT)
+        _check_fp32(phi, ref)
+        k = sv["part"]
+        reached = np.flatnonzero(ref < 1e50)
+        assert reached.min() >= bounds[k] and reached.max() < bounds[k + 1]
+    s.close()
