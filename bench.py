@@ -59,7 +59,7 @@ def traffic_from_profiles(args):
     text = open(p).read()
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
     c = re.search(r"traffic_config:\s*(.*)", text)
-    mine = (f"batch={args.batch} max_nn={args.max_nn} block={args.block} "
+    mine = (f"batch={args.batch} max_nn={args.max_nn} block={args.block} unit={args.unit} "
             f"window={float(args.window)} staged={args.staged}")
     src = re.search(r"traffic_source:\s*([0-9a-f]+)", text)
     if not m or not c or c.group(1).strip() != mine or not src or src.group(1) != kernel_source_digest():
@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--max-nn", type=int, default=1024)
     ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--unit", type=int, default=2,
+                    help="queued instances of one submesh run back to back by one workgroup")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--graph-steps", type=int, default=0,
@@ -128,7 +130,7 @@ def main():
             kern[k] += st[k]
 
     # ---- batched solver, timed -----------------------------------------
-    s = F.Solver(pack, n_instances=args.batch, block=args.block, profile=True,
+    s = F.Solver(pack, n_instances=args.batch, block=args.block, profile=True, unit=args.unit,
                  window=args.window, staged=args.staged)
     for i in range(args.warmup):
         solve_batch(s, sets)
@@ -155,7 +157,7 @@ def main():
     # to keep every compute launch of this command timed)
     graph_maps_per_s = None
     if args.graph_steps > 0:
-        g = F.Solver(pack, n_instances=args.batch, block=args.block, window=args.window,
+        g = F.Solver(pack, n_instances=args.batch, block=args.block, window=args.window, unit=args.unit,
                      staged=args.staged)
         solve_batch(g, sets)
         t0 = time.perf_counter()
@@ -229,6 +231,7 @@ def main():
             "partition": f"recursive bisection, <= {args.max_nn} vertex copies per submesh, "
                          f"{info['nsub']} submeshes",
             "workgroup": args.block,
+            "unit": args.unit,
             "window_us": args.window,
             "staged": args.staged,
         },

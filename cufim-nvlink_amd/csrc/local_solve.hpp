@@ -56,17 +56,19 @@ FIM_HD constexpr int edge_idx(int i, int j) {
 //     of (x1-x3, x2-x3), g = their products with x4-x3, d = (p1-p3, p2-p3),
 //     s^2 (1 - d^T G^-1 d) = |e34|^2 - g^T G^-1 g,  l = G^-1 (g - s d).
 // ---------------------------------------------------------------------------
+// Both solvers are evaluated without branches: every quantity is computed
+// and an invalid configuration (degenerate edge / face, ray leaving the
+// simplex, unusable node time) selects `inf` at the end.  Where the
+// configuration is valid the operations are exactly the sequential ones.
 template <class T>
 FIM_HD inline T tri_exact(T e12, T e14, T e24, T p1, T p2, T inf) {
   const T G = e12, g = T(0.5) * (e12 + e14 - e24), d = p2 - p1;
   const T den = G - d * d;
-  if (!(den > T(0))) return inf;
   const T r = e14 - cdiv(g * g, G);
-  if (!(r > T(0))) return inf;
-  const T s = csqrt(cdiv(r * G, den));
+  const T s = csqrt(std::fmax(cdiv(r * G, den), T(0)));
   const T l = cdiv(g - s * d, G);
-  if (!(l > T(0) && l < T(1))) return inf;
-  return p1 + l * d + s;
+  const bool ok = (den > T(0)) && (r > T(0)) && (l > T(0)) && (l < T(1));
+  return ok ? p1 + l * d + s : inf;
 }
 
 template <class T>
@@ -76,37 +78,42 @@ FIM_HD inline T tet_exact(T e12, T e13, T e23, T e14, T e24, T e34, T p1, T p2, 
   const T Au = T(0.5) * (e13 + e34 - e14), Bu = T(0.5) * (e23 + e34 - e24);
   const T d1 = p1 - p3, d2 = p2 - p3;
   const T det = AA * BB - AB * AB;
-  if (!(det > T(0))) return inf;
   const T w = cdiv(BB * d1 * d1 - T(2) * AB * d1 * d2 + AA * d2 * d2, det);
-  if (!(w < T(1))) return inf;
   const T r = e34 - cdiv(BB * Au * Au - T(2) * AB * Au * Bu + AA * Bu * Bu, det);
-  if (!(r > T(0))) return inf;
-  const T s = csqrt(cdiv(r, T(1) - w));
+  const T s = csqrt(std::fmax(cdiv(r, T(1) - w), T(0)));
   const T t1 = Au - s * d1, t2 = Bu - s * d2;
   const T l1 = cdiv(BB * t1 - AB * t2, det);
   const T l2 = cdiv(AA * t2 - AB * t1, det);
-  if (!(l1 > T(0) && l2 > T(0) && l1 + l2 < T(1))) return inf;
-  return p3 + l1 * d1 + l2 * d2 + s;
+  const bool ok = (det > T(0)) && (w < T(1)) && (r > T(0)) && (l1 > T(0)) && (l2 > T(0)) &&
+                  (l1 + l2 < T(1));
+  return ok ? p3 + l1 * d1 + l2 * d2 + s : inf;
 }
 
-// candidate for node position TP; ph[q] >= inf marks an unusable node
+// candidate of node 4 from nodes 1, 2, 3 (already relabelled); a time >= inf
+// marks an unusable node
+template <class T>
+FIM_HD inline T cand_core(T e12, T e13, T e23, T e14, T e24, T e34, T p1, T p2, T p3, T inf) {
+  const bool u1 = p1 < inf, u2 = p2 < inf, u3 = p3 < inf;
+  T best = std::fmin(std::fmin(u1 ? p1 + csqrt(e14) : inf, u2 ? p2 + csqrt(e24) : inf),
+                     u3 ? p3 + csqrt(e34) : inf);
+  const T t12 = tri_exact<T>(e12, e14, e24, p1, p2, inf);
+  const T t13 = tri_exact<T>(e13, e14, e34, p1, p3, inf);
+  const T t23 = tri_exact<T>(e23, e24, e34, p2, p3, inf);
+  const T t123 = tet_exact<T>(e12, e13, e23, e14, e24, e34, p1, p2, p3, inf);
+  best = std::fmin(best, (u1 && u2) ? t12 : inf);
+  best = std::fmin(best, (u1 && u3) ? t13 : inf);
+  best = std::fmin(best, (u2 && u3) ? t23 : inf);
+  best = std::fmin(best, (u1 && u2 && u3) ? t123 : inf);
+  return best;
+}
+
+// candidate for node position TP (compile time)
 template <int TP, class T>
 FIM_HD inline T cand_exact(const T g[6], const T ph[4], T inf) {
   constexpr int q1 = (TP + 1) & 3, q2 = (TP + 2) & 3, q3 = (TP + 3) & 3;
-  const T e12 = g[edge_idx(q1, q2)], e13 = g[edge_idx(q1, q3)], e23 = g[edge_idx(q2, q3)];
-  const T e14 = g[edge_idx(q1, TP)], e24 = g[edge_idx(q2, TP)], e34 = g[edge_idx(q3, TP)];
-  const T p1 = ph[q1], p2 = ph[q2], p3 = ph[q3];
-  const bool u1 = p1 < inf, u2 = p2 < inf, u3 = p3 < inf;
-  T best = inf;
-  if (u1) best = std::fmin(best, p1 + csqrt(e14));
-  if (u2) best = std::fmin(best, p2 + csqrt(e24));
-  if (u3) best = std::fmin(best, p3 + csqrt(e34));
-  if (u1 && u2) best = std::fmin(best, tri_exact<T>(e12, e14, e24, p1, p2, inf));
-  if (u1 && u3) best = std::fmin(best, tri_exact<T>(e13, e14, e34, p1, p3, inf));
-  if (u2 && u3) best = std::fmin(best, tri_exact<T>(e23, e24, e34, p2, p3, inf));
-  if (u1 && u2 && u3)
-    best = std::fmin(best, tet_exact<T>(e12, e13, e23, e14, e24, e34, p1, p2, p3, inf));
-  return best;
+  return cand_core<T>(g[edge_idx(q1, q2)], g[edge_idx(q1, q3)], g[edge_idx(q2, q3)],
+                      g[edge_idx(q1, TP)], g[edge_idx(q2, TP)], g[edge_idx(q3, TP)], ph[q1], ph[q2],
+                      ph[q3], inf);
 }
 
 // ---------------------------------------------------------------------------

@@ -57,7 +57,170 @@ __device__ inline bool flag_at(const uint32_t *bits, uint32_t l) {
 
 // ---------------------------------------------------------------------------
 // converge mode
+//
+// Work lists are bucketed by submesh: bcnt[s] instances of submesh s are
+// queued in bins[s * ninst ...].  A (instance, submesh) pair is queued at most
+// once per list (the act flag is set on queueing and cleared when the pair is
+// processed), so a bucket never holds more than ninst entries.  After the
+// list is complete the plan kernel cuts every bucket into work units of up to
+// `unit` instances (plan[u] = s * maxch + chunk, submesh-major) and writes the
+// unit count.  Units are dealt out XCD-aware: workgroup b runs on XCD b % 8,
+// so the unit range is split into 8 contiguous eighths and workgroups of XCD
+// x take units of eighth x first (one static unit each, then from the
+// eighth's counter), then help the other eighths.  Instances of one submesh
+// thus run on one XCD, back to back, and its element data is fetched into
+// that XCD's L2 once.
 // ---------------------------------------------------------------------------
+struct WorkList {
+  uint32_t *bcnt;   // nsub: queued instances per submesh
+  uint32_t *bins;   // nsub x ninst instance ids
+  uint32_t *plan;   // work units
+  uint32_t *cnt;    // [0] units, [1] queued pairs, [2..9] per-XCD unit counters
+};
+
+constexpr uint32_t kXcd = 8;
+
+__device__ inline void enqueue(const WorkList &l, uint32_t *act, uint32_t inst, uint32_t s,
+                               uint32_t nsub, uint32_t ninst) {
+  if (atomicExch(&act[(size_t)inst * nsub + s], 1u) == 0u) {
+    const uint32_t j = atomicAdd(&l.bcnt[s], 1u);
+    l.bins[(size_t)s * ninst + j] = inst;
+  }
+}
+
+struct UnitRange {
+  uint32_t b, e, nb;  // units [b, e); nb workgroups of the grid run on this XCD
+};
+
+__device__ inline UnitRange xcd_range(uint32_t U, uint32_t y, uint32_t G) {
+  UnitRange r;
+  r.b = (uint32_t)(((uint64_t)U * y) / kXcd);
+  r.e = (uint32_t)(((uint64_t)U * (y + 1)) / kXcd);
+  r.nb = G > y ? (G - y + kXcd - 1) / kXcd : 0u;
+  return r;
+}
+
+// Iterates the (instance, submesh) pairs of a list for one workgroup; every
+// thread of the workgroup calls next() together.  `slot` is one LDS word.
+struct UnitCursor {
+  const uint32_t *bcnt, *bins, *plan;
+  uint32_t *grab;
+  uint32_t *slot;
+  uint32_t U, G, x, y, d, ninst, unit, maxch;
+  UnitRange R;
+  uint32_t s, base, n_in, i;
+  bool first;
+
+  __device__ UnitCursor(const WorkList &l, uint32_t *lds_slot, uint32_t ninst_, uint32_t unit_,
+                        uint32_t maxch_)
+      : bcnt(l.bcnt), bins(l.bins), plan(l.plan), grab(l.cnt + 2), slot(lds_slot), U(l.cnt[0]),
+        G(gridDim.x), x(blockIdx.x % kXcd), y(blockIdx.x % kXcd), d(0), ninst(ninst_),
+        unit(unit_), maxch(maxch_), s(0), base(0), n_in(0), i(0), first(true) {
+    R = xcd_range(U, x, G);
+  }
+
+  __device__ bool next(uint32_t &inst, uint32_t &sub) {
+    if (++i < n_in) {
+      inst = bins[base + i];
+      sub = s;
+      return true;
+    }
+    for (;;) {
+      uint32_t u;
+      if (first) {
+        u = R.b + blockIdx.x / kXcd;
+        first = false;
+      } else {
+        // dynamic units follow the R.nb static ones; a plain read of the
+        // counter skips exhausted eighths without an atomic
+        if (threadIdx.x == 0) {
+          const uint32_t dyn = R.b + R.nb;
+          uint32_t v = 0xffffffffu;
+          if (dyn < R.e &&
+              dyn + __hip_atomic_load(&grab[y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < R.e)
+            v = dyn + atomicAdd(&grab[y], 1u);
+          *slot = v;
+        }
+        __syncthreads();
+        u = *slot;
+        __syncthreads();
+      }
+      if (u < R.e) {
+        const uint32_t pu = plan[u];
+        s = pu / maxch;
+        const uint32_t j = pu - s * maxch;
+        n_in = min(unit, bcnt[s] - j * unit);
+        base = s * ninst + j * unit;
+        i = 0;
+        inst = bins[base];
+        sub = s;
+        return true;
+      }
+      if (++d == kXcd) return false;
+      y = (x + d) % kXcd;
+      R = xcd_range(U, y, G);
+    }
+  }
+};
+
+// Builds the work units of a complete list (one workgroup of 1024 threads)
+// and clears the buckets of the list the iteration has consumed.
+struct PlanArgs {
+  const uint32_t *bcnt;
+  uint32_t *plan;
+  uint32_t *cnt;
+  uint32_t *zero_bcnt;  // consumed list, may be null
+  uint32_t nsub, unit, maxch;
+};
+
+__global__ __launch_bounds__(1024) void k_conv_plan(PlanArgs a) {
+  __shared__ uint32_t wsum[16], witems[16];
+  __shared__ uint32_t base_sh, items_sh;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) {
+    base_sh = 0u;
+    items_sh = 0u;
+  }
+  __syncthreads();
+  for (uint32_t s0 = 0; s0 < a.nsub; s0 += 1024) {
+    const uint32_t s = s0 + tid;
+    uint32_t n = 0;
+    if (s < a.nsub) {
+      n = a.bcnt[s];
+      if (a.zero_bcnt) a.zero_bcnt[s] = 0u;
+    }
+    const uint32_t ch = (n + a.unit - 1) / a.unit;
+    uint32_t v = ch, m = n;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t t = __shfl_up(v, off);
+      if (lane >= (uint32_t)off) v += t;
+      m += __shfl_xor(m, off);
+    }
+    if (lane == 63) wsum[wave] = v;
+    if (lane == 0) witems[wave] = m;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0, itot = 0;
+    for (uint32_t w = 0; w < 16; ++w) {
+      if (w < wave) pre += wsum[w];
+      tot += wsum[w];
+      itot += witems[w];
+    }
+    const uint32_t off = base_sh + pre + v - ch;
+    for (uint32_t j = 0; j < ch; ++j) a.plan[off + j] = s * a.maxch + j;
+    __syncthreads();
+    if (tid == 0) {
+      base_sh += tot;
+      items_sh += itot;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    a.cnt[0] = base_sh;
+    a.cnt[1] = items_sh;
+  }
+  if (tid < kXcd) a.cnt[2 + tid] = 0u;
+}
+
 struct ConvArgs {
   const SubInfo *sub;
   const uint2 *e2n;      // 4 x u16 local ids per element
@@ -70,15 +233,8 @@ struct ConvArgs {
   uint8_t *chg;          // n_inst x ncopy
   uint32_t *act;         // n_inst x nsub
   float *wake;           // n_inst x nsub: lower bound of the dirty vertex times
-  const uint32_t *list;  // current work list
-  const uint32_t *count;
-  uint32_t *next_list;   // next work list (requeue on the sweep cap)
-  uint32_t *next_count;  // filled by requeue here and by the sync kernel
-  uint32_t *zero_count;  // counter of the list after next: zeroed here.  It
-                         // was consumed by the previous iteration and is
-                         // appended to only from the next iteration on.
-  uint32_t *grab;        // work-item counter of the current list
-  uint32_t *zero_grab;   // work-item counter of the next list, zeroed here
+  WorkList cur;          // work list of this iteration
+  WorkList nxt;          // work list of the next iteration (requeues)
   unsigned long long *stats;  // activations, sweeps, element evaluations,
                               // iterations, algorithmic bytes
   const uint32_t *iter;  // global iteration counter (advanced by the sync kernel)
@@ -87,10 +243,11 @@ struct ConvArgs {
   const uint4 *erec;        // band kernel: 48 B per element (ElemRecord layout)
   const uint32_t *adj_ptr;  // band kernel: per copy, chunk offset << 6 | chunk count
   const uint4 *adj;         // 8 incident local element ids (u16) per chunk, 0xffff padded
-  uint32_t nsub, ncopy;
+  uint32_t nsub, ncopy, ninst;
   float eps;
   float window;          // per-iteration time window, 0 = off
   int32_t max_local;
+  uint32_t unit, maxch;  // instances per work unit, units per submesh bucket
   // LDS carve (bytes)
   uint32_t o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc, o_wrt;
 };
@@ -118,19 +275,15 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
   uint32_t *misc = reinterpret_cast<uint32_t *>(lds + a.o_misc);
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-  const uint32_t n_work = *a.count;
-  if (blockIdx.x == 0 && tid == 0) {
-    *a.zero_count = 0;
-    *a.zero_grab = 0;
-    if (n_work > 0) atomicAdd(&a.stats[3], 1ull);
-  }
+  UnitCursor cursor(a.cur, &misc[4], a.ninst, a.unit, a.maxch);
+  if (blockIdx.x == 0 && tid == 0 && cursor.U > 0) atomicAdd(&a.stats[3], 1ull);
   unsigned long long sweeps = 0, evals = 0, acts = 0, abytes = 0, vupd = 0;
   // time window of this iteration: flagged vertices later than W wait
   const float W = a.window > 0.0f ? a.window * (float)(*a.iter + 1u) : F_INF;
 
-  for (uint32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
-    const uint32_t entry = a.list[w];
-    const uint32_t inst = entry / a.nsub, s = entry - inst * a.nsub;
+  uint32_t inst, s;
+  while (cursor.next(inst, s)) {
+    const size_t entry = (size_t)inst * a.nsub + s;
     const SubInfo si = a.sub[s];
     const size_t cb = (size_t)inst * a.ncopy + si.coff;
     const uint32_t nn = si.nn, ne = si.ne;
@@ -140,10 +293,7 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
     // nothing in this submesh can act before the window reaches its wake
     // time: keep it queued without staging anything
     if (a.wake[entry] > W) {
-      if (tid == 0 && atomicExch(&a.act[entry], 1u) == 0u) {
-        const uint32_t pos = atomicAdd(a.next_count, 1u);
-        a.next_list[pos] = entry;
-      }
+      if (tid == 0) enqueue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
       continue;
     }
     // stage vertex times and dirty flags
@@ -396,10 +546,7 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
     if (tid == 0) {
       const float wk = __uint_as_float(misc[1]);
       a.wake[entry] = wk;
-      if (wk < F_INF && atomicExch(&a.act[entry], 1u) == 0u) {
-        const uint32_t pos = atomicAdd(a.next_count, 1u);
-        a.next_list[pos] = entry;
-      }
+      if (wk < F_INF) enqueue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
     }
     __syncthreads();
   }
@@ -421,19 +568,6 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
 // Local copy ids are below 2^15, so the two region bits are free.
 constexpr uint32_t kAdjPad = 0xffffffffu;
 
-__device__ inline void mark_word(uint32_t *emark, uint32_t w) {
-  const uint32_t lo = w & 0xffffu, hi = w >> 16;
-  if (lo != 0xffffu) atomicOr(&emark[lo >> 5], 1u << (lo & 31));
-  if (hi != 0xffffu) atomicOr(&emark[hi >> 5], 1u << (hi & 31));
-}
-
-__device__ inline void mark_chunk(uint32_t *emark, const uint4 q) {
-  mark_word(emark, q.x);
-  mark_word(emark, q.y);
-  mark_word(emark, q.z);
-  mark_word(emark, q.w);
-}
-
 // expands the set bits of `bits` (word k of a bitmask) into list[base...]
 __device__ inline void expand_bits(uint32_t bits, uint32_t k, uint16_t *list, uint32_t base) {
   while (bits) {
@@ -443,18 +577,72 @@ __device__ inline void expand_bits(uint32_t bits, uint32_t k, uint16_t *list, ui
   }
 }
 
+// marks local element le for the next sweep; the first marker appends it to
+// the element list
+__device__ inline void mark_append(uint32_t *emark, uint16_t *elist, uint32_t *cnt, uint32_t le) {
+  const uint32_t bit = 1u << (le & 31);
+  if (!(atomicOr(&emark[le >> 5], bit) & bit)) elist[atomicAdd(cnt, 1u)] = (uint16_t)le;
+}
+
+__device__ inline void mark_word(uint32_t *emark, uint16_t *elist, uint32_t *cnt, uint32_t w) {
+  const uint32_t lo = w & 0xffffu, hi = w >> 16;
+  if (lo != 0xffffu) mark_append(emark, elist, cnt, lo);
+  if (hi != 0xffffu) mark_append(emark, elist, cnt, hi);
+}
+
+__device__ inline void mark_chunk(uint32_t *emark, uint16_t *elist, uint32_t *cnt, const uint4 q) {
+  mark_word(emark, elist, cnt, q.x);
+  mark_word(emark, elist, cnt, q.y);
+  mark_word(emark, elist, cnt, q.z);
+  mark_word(emark, elist, cnt, q.w);
+}
+
+// marks the incident elements of copy c (global copy id): up to four 8-entry
+// chunks of the padded adjacency are loaded before any is used
+__device__ inline void mark_incident(const ConvArgs &a, uint32_t c, uint32_t *emark, uint16_t *elist,
+                                     uint32_t *cnt) {
+  const uint32_t ap = a.adj_ptr[c];
+  const uint4 *ch = a.adj + (ap >> 6);
+  const uint32_t nch = ap & 63u;
+  const uint4 pad = make_uint4(kAdjPad, kAdjPad, kAdjPad, kAdjPad);
+  const uint4 q0 = ch[0];
+  const uint4 q1 = nch > 1 ? ch[1] : pad;
+  const uint4 q2 = nch > 2 ? ch[2] : pad;
+  const uint4 q3 = nch > 3 ? ch[3] : pad;
+  mark_chunk(emark, elist, cnt, q0);
+  mark_chunk(emark, elist, cnt, q1);
+  mark_chunk(emark, elist, cnt, q2);
+  mark_chunk(emark, elist, cnt, q3);
+  for (uint32_t k = 4; k < nch; ++k) mark_chunk(emark, elist, cnt, ch[k]);
+}
+
+// appends the lanes with `pred` (vertex l) to list[*cnt ...], one LDS atomic
+// per wave
+__device__ inline void wave_append(bool pred, uint32_t l, uint16_t *list, uint32_t *cnt) {
+  const uint64_t m = __ballot(pred);
+  uint32_t base = 0;
+  if ((threadIdx.x & 63) == 0 && m) base = atomicAdd(cnt, (uint32_t)__popcll(m));
+  base = __shfl(base, 0);
+  if (pred) list[base + __popcll(m & lanes_below())] = (uint16_t)l;
+}
+
 // Unstaged converge kernel.  Per sweep it touches only the band that is
-// active: the flagged vertices mark their incident elements through the
-// vertex -> element adjacency (LDS bitmask), the marked elements are
-// evaluated from L2-resident data, LDS atomicMin winners set a "written"
-// bit, and only the written vertices are revisited to form the next
-// sweep's sources.  Element data never enters LDS.
+// active, in three steps separated by barriers:
+//   mark: the sources (a compact list) mark their incident elements through
+//     the vertex -> element adjacency; the first marker of an element
+//     appends it to the element list;
+//   evaluate: the listed elements compute candidates from L2-resident
+//     element records; LDS atomicMin winners set a "written" bit;
+//   accept: each written vertex takes its new time; a significant decrease
+//     (reference criterion, cuda_fim.cu:2154) makes it a source of the next
+//     sweep, or a deferred vertex if it lies beyond the window.
+// Element data never enters LDS.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_conv_band(ConvArgs a) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_conv_band(ConvArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   float *phi_l = reinterpret_cast<float *>(lds);
   uint32_t *phi_n = reinterpret_cast<uint32_t *>(lds + a.o_phin);
-  uint32_t *flg = reinterpret_cast<uint32_t *>(lds + a.o_flg);  // sources of this sweep
+  uint32_t *flg = reinterpret_cast<uint32_t *>(lds + a.o_flg);  // sources of the sweep
   uint32_t *tch = reinterpret_cast<uint32_t *>(lds + a.o_tch);  // decreased this activation
   uint32_t *dfr = reinterpret_cast<uint32_t *>(lds + a.o_dfr);  // flagged, beyond the window
   uint32_t *wrt = reinterpret_cast<uint32_t *>(lds + a.o_wrt);  // lowered in this sweep
@@ -462,131 +650,97 @@ __global__ __launch_bounds__(BS) void k_conv_band(ConvArgs a) {
   uint32_t *eseen = reinterpret_cast<uint32_t *>(lds + a.o_geo);
   uint16_t *elist = reinterpret_cast<uint16_t *>(lds + a.o_list);
   uint16_t *vlist = reinterpret_cast<uint16_t *>(lds + a.o_vlist);
+  // misc: [0] listed elements, [2] listed sources, [3] distinct elements,
+  // [4] work cursor, [5] earliest deferred time
   uint32_t *misc = reinterpret_cast<uint32_t *>(lds + a.o_misc);
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-  const uint32_t n_work = *a.count;
-  if (blockIdx.x == 0 && tid == 0) {
-    *a.zero_count = 0;
-    *a.zero_grab = 0;
-    if (n_work > 0) atomicAdd(&a.stats[3], 1ull);
-  }
+  UnitCursor cursor(a.cur, &misc[4], a.ninst, a.unit, a.maxch);
+  if (blockIdx.x == 0 && tid == 0 && cursor.U > 0) atomicAdd(&a.stats[3], 1ull);
   unsigned long long sweeps = 0, evals = 0, acts = 0, abytes = 0, vupd = 0;
   const float W = a.window > 0.0f ? a.window * (float)(*a.iter + 1u) : F_INF;
 
-  // the first item of each workgroup is its own index; later items are
-  // taken one at a time from the list's counter, so workgroups that drew
-  // cheap items take more of them (no atomic traffic while the list is
-  // shorter than the grid)
-  auto next_item = [&]() -> uint32_t {
-    if (tid == 0) misc[4] = gridDim.x + atomicAdd(a.grab, 1u);
-    __syncthreads();
-    const uint32_t nw = misc[4];
-    __syncthreads();
-    return nw;
-  };
-  for (uint32_t w = blockIdx.x; w < n_work; w = next_item()) {
-    const uint32_t entry = a.list[w];
-    const uint32_t inst = entry / a.nsub, s = entry - inst * a.nsub;
+  uint32_t inst, s;
+  while (cursor.next(inst, s)) {
+    const size_t entry = (size_t)inst * a.nsub + s;
     const SubInfo si = a.sub[s];
     const size_t cb = (size_t)inst * a.ncopy + si.coff;
     const uint32_t nn = si.nn, ne = si.ne;
     const uint32_t nv64 = (nn + 63u) & ~63u;
-    const uint32_t nw = nv64 >> 5;           // vertex bitmask words
     const uint32_t new32 = (ne + 31u) >> 5;  // element bitmask words
     if (tid == 0) a.act[entry] = 0u;
     if (a.wake[entry] > W) {
-      if (tid == 0 && atomicExch(&a.act[entry], 1u) == 0u) {
-        const uint32_t pos = atomicAdd(a.next_count, 1u);
-        a.next_list[pos] = entry;
-      }
+      if (tid == 0) enqueue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
       continue;
-    }
-    // stage vertex times; dirty flags split into sources and deferred
-    for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
-      const uint32_t l = l0 + tid;
-      bool d = false, late = false;
-      if (l < nn) {
-        const float v = a.phi[cb + l];
-        phi_l[l] = v;
-        phi_n[l] = __float_as_uint(v);
-        d = a.dirty[cb + l] != 0;
-        if (d) a.dirty[cb + l] = 0;
-        late = d && v > W;
-        d = d && !late;
-      }
-      const uint64_t m = __ballot(d), md = __ballot(late);
-      const uint32_t wb = l0 + wave * 64;
-      if (wb < nv64) {
-        if (lane == 0) flg[wb >> 5] = (uint32_t)m;
-        if (lane == 1) flg[(wb >> 5) + 1] = (uint32_t)(m >> 32);
-        if (lane == 2) tch[wb >> 5] = 0u;
-        if (lane == 3) tch[(wb >> 5) + 1] = 0u;
-        if (lane == 4) dfr[wb >> 5] = (uint32_t)md;
-        if (lane == 5) dfr[(wb >> 5) + 1] = (uint32_t)(md >> 32);
-        if (lane == 6) wrt[wb >> 5] = 0u;
-        if (lane == 7) wrt[(wb >> 5) + 1] = 0u;
-      }
     }
     for (uint32_t k = tid; k < new32; k += BS) {
       emark[k] = 0u;
       eseen[k] = 0u;
     }
-    if (tid == 0) misc[2] = 0u;
+    if (tid == 0) {
+      misc[0] = 0u;
+      misc[2] = 0u;
+      misc[3] = 0u;
+      misc[5] = __float_as_uint(F_INF);
+    }
     __syncthreads();
-    // initial source list
-    for (uint32_t k = tid; k < nw; k += BS) {
-      const uint32_t bits = flg[k];
-      if (bits) expand_bits(bits, k, vlist, atomicAdd(&misc[2], (uint32_t)__popc(bits)));
+    // stage vertex times; dirty flags split into sources and deferred
+    for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
+      const uint32_t l = l0 + tid;
+      const uint32_t wb = l0 + wave * 64;
+      if (wb < nv64) {
+        bool d = false, late = false;
+        if (l < nn) {
+          const float v = a.phi[cb + l];
+          phi_l[l] = v;
+          phi_n[l] = __float_as_uint(v);
+          d = a.dirty[cb + l] != 0;
+          if (d) a.dirty[cb + l] = 0;
+          late = d && v > W;
+          d = d && !late;
+        }
+        const uint64_t m = __ballot(d), md = __ballot(late);
+        const uint32_t k = wb >> 5;
+        if (lane == 0) {
+          flg[k] = (uint32_t)m;
+          flg[k + 1] = (uint32_t)(m >> 32);
+          dfr[k] = (uint32_t)md;
+          dfr[k + 1] = (uint32_t)(md >> 32);
+        }
+        if (lane == 1) {
+          tch[k] = 0u;
+          tch[k + 1] = 0u;
+          wrt[k] = 0u;
+          wrt[k + 1] = 0u;
+        }
+        wave_append(d, l, vlist, &misc[2]);
+      }
     }
     const float *rs = a.rscale + 4 * (size_t)inst;
     __syncthreads();
 
-    int it = 0;
+    uint32_t it = 0;
     bool capped = false;
     for (;;) {
       const uint32_t nsrc = misc[2];
       if (nsrc == 0) break;
-      if ((a.max_local > 0 && it >= a.max_local) || it >= kHardLocalCap) {
+      if ((a.max_local > 0 && it >= (uint32_t)a.max_local) || it >= (uint32_t)kHardLocalCap) {
         capped = true;
         break;
       }
       ++it;
-      // mark the elements incident to the sources: up to four 8-entry chunks
-      // of the padded adjacency are loaded before any is used
-      for (uint32_t i = tid; i < nsrc; i += BS) {
-        const uint32_t c = si.coff + vlist[i];
-        const uint32_t ap = a.adj_ptr[c];
-        const uint4 *ch = a.adj + (ap >> 6);
-        const uint32_t nch = ap & 63u;
-        const uint4 pad = make_uint4(kAdjPad, kAdjPad, kAdjPad, kAdjPad);
-        const uint4 q0 = ch[0];
-        const uint4 q1 = nch > 1 ? ch[1] : pad;
-        const uint4 q2 = nch > 2 ? ch[2] : pad;
-        const uint4 q3 = nch > 3 ? ch[3] : pad;
-        mark_chunk(emark, q0);
-        mark_chunk(emark, q1);
-        mark_chunk(emark, q2);
-        mark_chunk(emark, q3);
-        for (uint32_t k = 4; k < nch; ++k) mark_chunk(emark, ch[k]);
-      }
-      if (tid == 0) misc[0] = 0u;
-      __syncthreads();
-      // marked elements -> list (marks cleared)
-      for (uint32_t k = tid; k < new32; k += BS) {
-        const uint32_t bits = emark[k];
-        if (bits) {
-          emark[k] = 0u;
-          eseen[k] |= bits;
-          expand_bits(bits, k, elist, atomicAdd(&misc[0], (uint32_t)__popc(bits)));
-        }
-      }
+      // mark the elements incident to the sources
+      for (uint32_t i = tid; i < nsrc; i += BS) mark_incident(a, si.coff + vlist[i], emark, elist, &misc[0]);
       __syncthreads();
       const uint32_t nact = misc[0];
       evals += nact;
-      // candidates of the marked elements
+      if (tid == 0) misc[2] = 0u;  // every thread has read the source count
+      // candidates of the listed elements
       for (uint32_t i = tid; i < nact; i += BS) {
         const uint32_t e = elist[i];
+        const uint32_t ebit = 1u << (e & 31);
+        atomicAnd(&emark[e >> 5], ~ebit);
+        atomicOr(&eseen[e >> 5], ebit);
         const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
         const uint4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
         const float sc = rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
@@ -606,8 +760,8 @@ __global__ __launch_bounds__(BS) void k_conv_band(ConvArgs a) {
           f[k] = flag_at(flg, nd[k]) ? 1 : 0;
         }
         const int nf = f[0] + f[1] + f[2] + f[3];
-        // targets with a flagged other node whose bound (earliest other node
-        // + metric height) is still below their time
+        // targets with a source among the other nodes whose bound (earliest
+        // other node + metric height) is still below their time
 #define FIM_TARGET(T, A, B, C)                                                        \
         if (nf - f[T] > 0 && ph[T] > fminf(ph[A], fminf(ph[B], ph[C])) + h[T]) {         \
           const float c = cand_exact<T, float>(g, ph, F_INF);                            \
@@ -624,54 +778,54 @@ __global__ __launch_bounds__(BS) void k_conv_band(ConvArgs a) {
         FIM_TARGET(3, 0, 1, 2)
 #undef FIM_TARGET
       }
-      if (tid == 0) misc[2] = 0u;
       __syncthreads();
-      // accept the written vertices: significant decreases (reference
-      // criterion, cuda_fim.cu:2154) become the next sources or, beyond the
-      // window, deferred flags
-      for (uint32_t k = tid; k < nw; k += BS) {
-        uint32_t bits = wrt[k];
-        uint32_t sg = 0u, dc = 0u, lt = 0u;
-        if (bits) {
-          wrt[k] = 0u;
-          while (bits) {
-            const uint32_t b = __ffs(bits) - 1;
-            bits &= bits - 1;
-            const uint32_t l = k * 32 + b;
+      if (tid == 0) misc[0] = 0u;  // every thread has read the element count
+      // accept the written vertices: the sources of the next sweep
+      for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
+        const uint32_t l = l0 + tid;
+        const uint32_t wb = l0 + wave * 64;
+        if (wb < nv64) {
+          bool sg = false, lt = false, dc = false;
+          const uint32_t k = wb >> 5;
+          if (l < nn && ((wrt[l >> 5] >> (l & 31)) & 1u)) {
             const float ov = phi_l[l];
             const float nv = __uint_as_float(phi_n[l]);
             if (nv < ov) {
               phi_l[l] = nv;
-              dc |= 1u << b;
+              dc = true;
               if ((ov - nv) >= a.eps * (1.0f + 0.5f * (ov + nv))) {
-                if (nv > W)
-                  lt |= 1u << b;
-                else
-                  sg |= 1u << b;
+                lt = nv > W;
+                sg = !lt;
               }
             }
           }
-          tch[k] |= dc;
+          const uint64_t msg = __ballot(sg), mlt = __ballot(lt), mdc = __ballot(dc);
+          if (lane == 0) {
+            flg[k] = (uint32_t)msg;
+            flg[k + 1] = (uint32_t)(msg >> 32);
+            wrt[k] = 0u;
+            wrt[k + 1] = 0u;
+          }
+          if (lane == 1) {
+            tch[k] |= (uint32_t)mdc;
+            tch[k + 1] |= (uint32_t)(mdc >> 32);
+          }
+          if (lane == 2) {
+            dfr[k] = (dfr[k] & ~(uint32_t)msg) | (uint32_t)mlt;
+            dfr[k + 1] = (dfr[k + 1] & ~(uint32_t)(msg >> 32)) | (uint32_t)(mlt >> 32);
+          }
+          wave_append(sg, l, vlist, &misc[2]);
         }
-        flg[k] = sg;
-        dfr[k] = (dfr[k] & ~sg) | lt;
-        if (sg) expand_bits(sg, k, vlist, atomicAdd(&misc[2], (uint32_t)__popc(sg)));
       }
       __syncthreads();
     }
     sweeps += it;
     acts += 1;
     // algorithmic bytes: copies (phi 4+4, flag 1+1), interface entries
-    // (4+4+4) and the distinct elements used (nodes 8 + geometry 24 +
-    // heights 16)
-    if (tid == 0) misc[3] = 0u;
-    __syncthreads();
+    // (4+4+4) and the distinct elements evaluated (48 B record)
     uint32_t cnt = 0;
     for (uint32_t k = tid; k < new32; k += BS) cnt += __popc(eseen[k]);
     if (cnt) atomicAdd(&misc[3], cnt);
-    if (tid == 0) misc[1] = __float_as_uint(F_INF);
-    __syncthreads();
-    abytes += 48ull * misc[3] + 10ull * nn + 12ull * si.ni;
     // write back decreased copies; deferred vertices (and sources left by
     // the sweep cap) stay dirty and the submesh is queued again
     for (uint32_t l = tid; l < nn; l += BS) {
@@ -681,17 +835,15 @@ __global__ __launch_bounds__(BS) void k_conv_band(ConvArgs a) {
       }
       if (flag_at(dfr, l) || (capped && flag_at(flg, l))) {
         a.dirty[cb + l] = 1;
-        atomicMin(&misc[1], __float_as_uint(phi_l[l]));
+        atomicMin(&misc[5], __float_as_uint(phi_l[l]));
       }
     }
     __syncthreads();
     if (tid == 0) {
-      const float wk = __uint_as_float(misc[1]);
+      abytes += 48ull * misc[3] + 10ull * nn + 12ull * si.ni;
+      const float wk = __uint_as_float(misc[5]);
       a.wake[entry] = wk;
-      if (wk < F_INF && atomicExch(&a.act[entry], 1u) == 0u) {
-        const uint32_t pos = atomicAdd(a.next_count, 1u);
-        a.next_list[pos] = entry;
-      }
+      if (wk < F_INF) enqueue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
     }
     __syncthreads();
   }
@@ -717,42 +869,42 @@ struct SyncArgs {
   uint8_t *chg;
   uint32_t *act;
   float *wake;
-  const uint32_t *list;
-  const uint32_t *count;
-  uint32_t *next_list;
-  uint32_t *next_count;
+  WorkList cur, nxt;
   uint32_t *iter;
-  uint32_t nsub, ncopy;
+  uint32_t nsub, ncopy, ninst, unit, maxch;
 };
 
 // interface push: min over copies, order independent
 __global__ __launch_bounds__(256) void k_conv_sync(SyncArgs a) {
-  const uint32_t n_work = *a.count;
   // the compute kernel of this iteration has finished reading the counter
   if (blockIdx.x == 0 && threadIdx.x == 0) *a.iter += 1u;
-  for (uint32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
-    const uint32_t entry = a.list[w];
-    const uint32_t inst = entry / a.nsub, s = entry - inst * a.nsub;
+  // units of this workgroup's XCD eighth, each once (the grid has >= 8
+  // workgroups, so every eighth is covered)
+  const UnitRange R = xcd_range(a.cur.cnt[0], blockIdx.x % kXcd, gridDim.x);
+  for (uint32_t u = R.b + blockIdx.x / kXcd; u < R.e; u += R.nb) {
+    const uint32_t pu = a.cur.plan[u];
+    const uint32_t s = pu / a.maxch, ch = pu - s * a.maxch;
+    const uint32_t n_in = min(a.unit, a.cur.bcnt[s] - ch * a.unit);
     const SubInfo si = a.sub[s];
-    const size_t ib = (size_t)inst * a.ncopy;
-    for (uint32_t j = threadIdx.x; j < si.ni; j += blockDim.x) {
-      const uint32_t idx = si.ioff + j;
-      const size_t gc = ib + si.coff + a.if_l[idx];
-      if (!a.chg[gc]) continue;
-      a.chg[gc] = 0;
-      const uint32_t v = __float_as_uint(a.phi[gc]);
-      for (uint32_t k = a.if_dsp[idx]; k < a.if_dsp[idx + 1]; ++k) {
-        const uint32_t oc = a.if_other[k];
-        uint32_t *p = reinterpret_cast<uint32_t *>(a.phi + ib + oc);
-        if (v < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-          const uint32_t old = atomicMin(p, v);
-          if (v < old) {
-            a.dirty[ib + oc] = 1;
-            const uint32_t e2 = inst * a.nsub + a.c2s[oc];
-            atomicMin(reinterpret_cast<uint32_t *>(a.wake + e2), v);
-            if (atomicExch(&a.act[e2], 1u) == 0u) {
-              const uint32_t pos = atomicAdd(a.next_count, 1u);
-              a.next_list[pos] = e2;
+    for (uint32_t q = 0; q < n_in; ++q) {
+      const uint32_t inst = a.cur.bins[(size_t)s * a.ninst + ch * a.unit + q];
+      const size_t ib = (size_t)inst * a.ncopy;
+      for (uint32_t j = threadIdx.x; j < si.ni; j += blockDim.x) {
+        const uint32_t idx = si.ioff + j;
+        const size_t gc = ib + si.coff + a.if_l[idx];
+        if (!a.chg[gc]) continue;
+        a.chg[gc] = 0;
+        const uint32_t v = __float_as_uint(a.phi[gc]);
+        for (uint32_t k = a.if_dsp[idx]; k < a.if_dsp[idx + 1]; ++k) {
+          const uint32_t oc = a.if_other[k];
+          uint32_t *p = reinterpret_cast<uint32_t *>(a.phi + ib + oc);
+          if (v < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            const uint32_t old = atomicMin(p, v);
+            if (v < old) {
+              a.dirty[ib + oc] = 1;
+              const uint32_t s2 = a.c2s[oc];
+              atomicMin(reinterpret_cast<uint32_t *>(a.wake + (size_t)inst * a.nsub + s2), v);
+              enqueue(a.nxt, a.act, inst, s2, a.nsub, a.ninst);
             }
           }
         }
@@ -765,16 +917,16 @@ __global__ __launch_bounds__(256) void k_conv_sync(SyncArgs a) {
 // their submesh is queued on the current list
 __global__ void k_conv_seed(const uint32_t *copies, uint32_t n, uint32_t inst, float t0,
                             const uint32_t *c2s, float *phi, uint8_t *dirty, uint32_t *act,
-                            float *wake, uint32_t *list, uint32_t *count, uint32_t nsub,
-                            uint32_t ncopy) {
+                            float *wake, WorkList l0, uint32_t nsub, uint32_t ncopy,
+                            uint32_t ninst) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t c = copies[i];
     const size_t gc = (size_t)inst * ncopy + c;
     atomicMin(reinterpret_cast<uint32_t *>(phi + gc), __float_as_uint(t0));
     dirty[gc] = 1;
-    const uint32_t e = inst * nsub + c2s[c];
-    atomicMin(reinterpret_cast<uint32_t *>(wake + e), __float_as_uint(t0));
-    if (atomicExch(&act[e], 1u) == 0u) list[atomicAdd(count, 1u)] = e;
+    const uint32_t s = c2s[c];
+    atomicMin(reinterpret_cast<uint32_t *>(wake + (size_t)inst * nsub + s), __float_as_uint(t0));
+    enqueue(l0, act, inst, s, nsub, ninst);
   }
 }
 
@@ -789,12 +941,11 @@ __global__ void k_conv_init(const float *phi0, const uint32_t *c2g, uint32_t ins
   }
 }
 
-__global__ void k_conv_queue_all(uint32_t inst, uint32_t nsub, uint32_t *act, float *wake,
-                                 uint32_t *list, uint32_t *count) {
+__global__ void k_conv_queue_all(uint32_t inst, uint32_t nsub, uint32_t ninst, uint32_t *act,
+                                 float *wake, WorkList l0) {
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nsub; s += gridDim.x * blockDim.x) {
-    const uint32_t e = inst * nsub + s;
-    wake[e] = 0.0f;
-    if (atomicExch(&act[e], 1u) == 0u) list[atomicAdd(count, 1u)] = e;
+    wake[(size_t)inst * nsub + s] = 0.0f;
+    enqueue(l0, act, inst, s, nsub, ninst);
   }
 }
 
@@ -1124,7 +1275,9 @@ struct fim_ctx {
   // converge state
   float *d_phi = nullptr, *d_rscale = nullptr;
   uint8_t *d_dirty = nullptr, *d_chg = nullptr;
-  uint32_t *d_act = nullptr, *d_list[3] = {nullptr, nullptr, nullptr}, *d_cnt = nullptr;
+  uint32_t *d_act = nullptr, *d_cnt = nullptr;
+  WorkList wl[3] = {};
+  uint32_t unit = 2, maxch = 1;
   float *d_wake = nullptr;
   // compat state
   double *d_phi64 = nullptr;
@@ -1205,28 +1358,31 @@ void launch_compat_compute(fim_ctx *c, const CompatArgs &a) {
 
 int blocks_for(size_t n, int bs) { return (int)std::max<size_t>(1, std::min<size_t>((n + bs - 1) / bs, 65536)); }
 
+// work units of list l; the buckets of list z (consumed), if any, are cleared
+void launch_plan(fim_ctx *c, int l, int z) {
+  PlanArgs pa{c->wl[l].bcnt, c->wl[l].plan, c->wl[l].cnt, z >= 0 ? c->wl[z].bcnt : nullptr,
+              c->nsub, c->unit, c->maxch};
+  k_conv_plan<<<1, 1024, 0, c->stream>>>(pa);
+  HIPCHECK(hipGetLastError());
+}
+
 // one global iteration k: work lists rotate over three buffers; iteration k
-// reads list k%3, appends to (k+1)%3 and zeroes the counter of (k+2)%3
+// reads list k%3 and appends to (k+1)%3, whose units are planned after the
+// interface push; the buckets of list k%3 are cleared then
 void enqueue_iteration(fim_ctx *c, int k, bool timed) {
-  const int p = k % 3, q = (k + 1) % 3, z = (k + 2) % 3;
+  const int p = k % 3, q = (k + 1) % 3;
   if (timed) HIPCHECK(hipEventRecord(c->kev[2 * k], c->stream));
   ConvArgs a = c->ca;
-  a.list = c->d_list[p];
-  a.count = c->d_cnt + p;
-  a.next_list = c->d_list[q];
-  a.next_count = c->d_cnt + q;
-  a.zero_count = c->d_cnt + z;
-  a.grab = c->d_cnt + 4 + p;
-  a.zero_grab = c->d_cnt + 4 + q;
+  a.cur = c->wl[p];
+  a.nxt = c->wl[q];
   launch_conv_compute(c, a);
   if (timed) HIPCHECK(hipEventRecord(c->kev[2 * k + 1], c->stream));
   SyncArgs s = c->sa;
-  s.list = c->d_list[p];
-  s.count = c->d_cnt + p;
-  s.next_list = c->d_list[q];
-  s.next_count = c->d_cnt + q;
+  s.cur = c->wl[p];
+  s.nxt = c->wl[q];
   k_conv_sync<<<c->grid, 256, 0, c->stream>>>(s);
   HIPCHECK(hipGetLastError());
+  launch_plan(c, q, p);
 }
 
 void build_graph(fim_ctx *c) {
@@ -1409,27 +1565,42 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         c->d_chg = dalloc<uint8_t>(ncs, ow);
         c->d_act = dalloc<uint32_t>(nss, ow);
         c->d_wake = dalloc<float>(nss, ow);
-        for (int k = 0; k < 3; ++k) c->d_list[k] = dalloc<uint32_t>(nss, ow);
-        // three list counters, the iteration counter, three work-item counters
-        c->d_cnt = dalloc<uint32_t>(8, ow);
+        c->unit = o->unit > 0 ? (uint32_t)o->unit : 2u;
+        c->maxch = (c->ninst + c->unit - 1) / c->unit;
+        if ((uint64_t)c->nsub * c->maxch >= (1ull << 32)) throw Error(FIM_ELIMIT, "too many work units");
+        // 16 words per list (units, pairs, 8 XCD counters), then the
+        // iteration counter at word 48
+        c->d_cnt = dalloc<uint32_t>(64, ow);
+        for (int k = 0; k < 3; ++k) {
+          c->wl[k].bcnt = dalloc<uint32_t>(c->nsub, ow);
+          c->wl[k].bins = dalloc<uint32_t>(nss, ow);
+          c->wl[k].plan = dalloc<uint32_t>((size_t)c->nsub * c->maxch, ow);
+          c->wl[k].cnt = c->d_cnt + 16 * k;
+        }
         c->d_rscale = dalloc<float>(4 * (size_t)c->ninst, ow);
         c->d_tmpf = dalloc<float>(c->nn, ow);
         k_fill<float><<<blocks_for(4 * (size_t)c->ninst, 256), 256, 0, c->stream>>>(c->d_rscale, 4 * (size_t)c->ninst, 1.0f);
         int ncu = 256;
         HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
-        const int per_cu = std::max(1, (int)((160 * 1024) / std::max<uint32_t>(c->lds_conv, 1)));
-        c->grid = o->grid > 0 ? o->grid : ncu * std::min(per_cu, 2048 / c->block);
+        // one resident wave of workgroups: registers, LDS and wave slots
+        set_lds_attr(c);
+        int per_cu = 0;
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, conv_kernel_ptr(o->staged != 0, c->block), c->block, c->lds_conv));
+        per_cu = std::max(per_cu, 1);
+        c->grid = o->grid > 0 ? o->grid : ncu * per_cu;
+        c->grid = std::max(c->grid, (int)kXcd);  // every XCD eighth needs a workgroup
         c->lds_compat = 0;
         c->ca = ConvArgs{c->d_sub, c->d_e2n, c->d_geo32, c->d_hgt, c->d_ereg, c->d_rscale, c->d_phi,
-                         c->d_dirty, c->d_chg, c->d_act, c->d_wake, nullptr, nullptr, nullptr,
-                         nullptr, nullptr, nullptr, nullptr, c->d_stats, c->d_cnt + 3,
+                         c->d_dirty, c->d_chg, c->d_act, c->d_wake, WorkList{}, WorkList{},
+                         c->d_stats, c->d_cnt + 48,
                          c->d_n2e_dsp, c->d_n2e_le, c->d_erec, c->d_adj_ptr, c->d_adj,
-                         c->nsub, c->ncopy, (float)(o->eps_rel > 0 ? o->eps_rel : 1e-6),
-                         (float)o->window, o->local_iters, o_phin, o_flg, o_tch, o_dfr, o_e2n,
+                         c->nsub, c->ncopy, c->ninst, (float)(o->eps_rel > 0 ? o->eps_rel : 1e-6),
+                         (float)o->window, o->local_iters, c->unit, c->maxch, o_phin, o_flg, o_tch, o_dfr, o_e2n,
                          o_geo, o_list, o_vlist, o_misc, o_wrt};
         c->sa = SyncArgs{c->d_sub, c->d_if_l, c->d_if_dsp, c->d_if_other, c->d_c2s, c->d_phi,
-                         c->d_dirty, c->d_chg, c->d_act, c->d_wake, nullptr, nullptr, nullptr,
-                         nullptr, c->d_cnt + 3, c->nsub, c->ncopy};
+                         c->d_dirty, c->d_chg, c->d_act, c->d_wake, WorkList{}, WorkList{},
+                         c->d_cnt + 48, c->nsub, c->ncopy, c->ninst, c->unit, c->maxch};
       } else {
         const uint32_t nn = (uint32_t)c->nn_max;
         const uint32_t o_tphi = al16(8 * nn);
@@ -1487,7 +1658,9 @@ int fim_reset(fim_ctx *c) {
       k_fill<uint8_t><<<blocks_for(ncs, 256), 256, 0, c->stream>>>(c->d_chg, ncs, (uint8_t)0);
       k_fill<uint32_t><<<blocks_for(nss, 256), 256, 0, c->stream>>>(c->d_act, nss, 0u);
       k_fill<float><<<blocks_for(nss, 256), 256, 0, c->stream>>>(c->d_wake, nss, F_INF);
-      k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_cnt, 8, 0u);
+      k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_cnt, 64, 0u);
+      for (int k = 0; k < 3; ++k)
+        k_fill<uint32_t><<<blocks_for(c->nsub, 256), 256, 0, c->stream>>>(c->wl[k].bcnt, c->nsub, 0u);
     } else {
       k_fill<double><<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_phi64, c->ncopy, DBL_MAX);
       k_fill<int8_t><<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_st, c->ncopy, (int8_t)FIM_FAR);
@@ -1513,8 +1686,7 @@ int fim_seed(fim_ctx *c, int32_t inst, const int64_t *vtx, int64_t n, double t0)
         if (!(t0 >= 0.0)) throw Error(FIM_EINVAL, "converge mode needs t0 >= 0");
         k_conv_seed<<<g, 256, 0, c->stream>>>(d, (uint32_t)cp.size(), (uint32_t)inst, (float)t0,
                                               c->d_c2s, c->d_phi, c->d_dirty, c->d_act,
-                                              c->d_wake, c->d_list[0], c->d_cnt, c->nsub,
-                                              c->ncopy);
+                                              c->d_wake, c->wl[0], c->nsub, c->ncopy, c->ninst);
       } else {
         k_compat_seed<<<g, 256, 0, c->stream>>>(d, (uint32_t)cp.size(), t0, c->d_phi64, c->d_st);
       }
@@ -1555,8 +1727,8 @@ int fim_init_field(fim_ctx *c, int32_t inst, const float *phi0) {
     HIPCHECK(hipMemcpyAsync(c->d_tmpf, phi0, c->nn * sizeof(float), hipMemcpyHostToDevice, c->stream));
     k_conv_init<<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_tmpf, c->d_c2g, (uint32_t)inst,
                                                                    c->d_phi, c->d_dirty, c->ncopy);
-    k_conv_queue_all<<<blocks_for(c->nsub, 256), 256, 0, c->stream>>>((uint32_t)inst, c->nsub, c->d_act,
-                                                                       c->d_wake, c->d_list[0], c->d_cnt);
+    k_conv_queue_all<<<blocks_for(c->nsub, 256), 256, 0, c->stream>>>((uint32_t)inst, c->nsub, c->ninst,
+                                                                       c->d_act, c->d_wake, c->wl[0]);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(c->stream));
   });
@@ -1604,6 +1776,8 @@ int fim_run(fim_ctx *c, int32_t max_iters, int32_t *iters) {
     if (!c->o.profile && !c->gexec) build_graph(c);
     HIPCHECK(hipMemsetAsync(c->d_stats, 0, 8 * sizeof(unsigned long long), c->stream));
     HIPCHECK(hipEventRecord(c->ev0, c->stream));
+    // units of the list the seeds (or an earlier capped run) filled
+    launch_plan(c, 0, -1);
     int32_t done = 0;
     uint32_t left = 1;
     double kms = 0.0;

@@ -13,7 +13,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(_HERE)
-LIB_PATH = os.path.join(PKG, "lib", "libcufim.so")
+# CUFIM_LIB selects another build of the library (same ABI)
+LIB_PATH = os.environ.get("CUFIM_LIB") or os.path.join(PKG, "lib", "libcufim.so")
 CLI_PATH = os.path.join(PKG, "bin", "cufim")
 
 FAR, NEIG, NARROW, NEXT, FROZEN, TAIL = range(6)
@@ -55,7 +56,7 @@ class Opts(C.Structure):
                 ("device", C.c_int32), ("block", C.c_int32), ("local_iters", C.c_int32),
                 ("graph_iters", C.c_int32), ("grid", C.c_int32), ("eps_rel", C.c_double),
                 ("apd", C.c_double), ("profile", C.c_int32), ("staged", C.c_int32),
-                ("window", C.c_double)]
+                ("window", C.c_double), ("unit", C.c_int32)]
 
 
 class CfgOptions(C.Structure):
@@ -241,12 +242,12 @@ class Pack:
 class Solver:
     def __init__(self, pack: Pack, mode=MODE_CONVERGE, local=LOCAL_EXACT, n_instances=1, device=0,
                  block=256, local_iters=0, graph_iters=0, grid=0, eps_rel=0.0, apd=0.0,
-                 profile=False, window=0.0, staged=False):
+                 profile=False, window=0.0, staged=False, unit=0):
         self.pack = pack
         self.nn = pack.info["nn_global"]
         self.n_instances = n_instances
         o = Opts(mode, local, n_instances, device, block, local_iters, graph_iters, grid, eps_rel, apd,
-                 int(profile), int(staged), window)
+                 int(profile), int(staged), window, unit)
         self.h = _vp()
         _check(lib().fim_create(pack.h, C.byref(o), C.byref(self.h)))
 

@@ -162,10 +162,11 @@ typedef struct {
                             unless set to EXACT)                            */
   int32_t n_instances;   /* independent solves on the pack (converge mode)   */
   int32_t device;        /* HIP device ordinal                               */
-  int32_t block;         /* workgroup size: 64, 128, 256 (0 = 256)           */
+  int32_t block;         /* workgroup size: 64, 128, 256, 512 (0 = 256)      */
   int32_t local_iters;   /* max local iterations per activation (0 = until
                             locally converged)                              */
-  int32_t graph_iters;   /* global iterations per captured graph (0 = 8)     */
+  int32_t graph_iters;   /* global iterations per captured graph, rounded up
+                            to a multiple of 3 (0 = 6)                      */
   int32_t grid;          /* workgroups of the persistent kernels (0 = auto)  */
   double eps_rel;        /* convergence threshold (0 = 1e-6, reference)      */
   double apd;            /* compat: action potential duration, us (0 = 3e5)  */
@@ -181,6 +182,10 @@ typedef struct {
                             as a source until the window passes it (0 = off,
                             plain FIM).  The fixed point does not depend on
                             it.                                             */
+  int32_t unit;          /* converge: queued instances of one submesh that
+                            one workgroup runs back to back as a work unit
+                            (0 = 2); the units of a list are dealt out per
+                            XCD so a submesh's element data stays in one L2 */
 } fim_opts;
 
 typedef struct {

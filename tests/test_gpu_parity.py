@@ -155,6 +155,30 @@ def test_converge_deterministic_and_batched():
     b.close()
 
 
+@pytest.mark.parametrize("unit", [1, 3, 16])
+@pytest.mark.parametrize("grid", [0, 8, 13])
+def test_converge_work_units_and_grid_change_nothing(unit, grid):
+    """work units of 1..16 instances, dealt to grids of 8, 13 or all
+    workgroups, give bitwise the same maps (12 instances, uneven buckets)"""
+    cfg = cases.scaled(cases.SLAB, 8)
+    m, _ = cases.box(cfg, seed=9)
+    p = F.Pack(m, F.PART_RCB, max_nn=256, seed=9)
+    srcs = [cases.pacing_sites(cfg, 1 + k % 3, seed=200 + k) for k in range(12)]
+
+    def solve(**kw):
+        s = F.Solver(p, n_instances=12, window=500.0, **kw)
+        for i, src in enumerate(srcs):
+            s.seed(src, 0.0, inst=i)
+        s.run()
+        out = [s.download(inst=i)[0] for i in range(12)]
+        s.close()
+        return out
+
+    base = solve(unit=4)
+    for a, b in zip(base, solve(unit=unit, grid=grid)):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("staged", [False, True])
 @pytest.mark.parametrize("window", [300.0, 1000.0, 4000.0])
 def test_converge_time_window_same_fixed_point(window, staged):
