@@ -28,6 +28,7 @@ HIPFLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "
 
 SOURCES = [
     ("solver.hip", True),
+    ("surrogate.hip", True),
     ("mesh.cpp", False),
     ("partition.cpp", False),
     ("host_capi.cpp", False),

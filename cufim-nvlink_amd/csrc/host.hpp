@@ -116,6 +116,21 @@ Pack build_pack(const Mesh &m, const Partition &p, int32_t source);
 // dynamic LDS bytes the converge-mode kernel needs for the largest submesh
 int64_t converge_lds_bytes(int32_t nn_max, int32_t ne_max);
 
+// solver-context hooks for the surrogate (csrc/surrogate.hip); device
+// pointers, work queued on the context's stream
+void *ctx_stream(fim_ctx *c);
+int32_t ctx_device(const fim_ctx *c);
+uint32_t ctx_vertices(const fim_ctx *c);
+int32_t ctx_instances(const fim_ctx *c);
+bool ctx_converge(const fim_ctx *c);
+// per-vertex map of an instance (min over copies) in the context's field buffer
+const float *ctx_gather(fim_ctx *c, int32_t inst);
+// the context's field buffer (nn floats): fill it, then ctx_init_field
+float *ctx_field_buffer(fim_ctx *c);
+// instance field := field buffer; all_active marks every copy dirty and
+// queues every submesh (warm start), otherwise only the times are set
+void ctx_init_field(fim_ctx *c, int32_t inst, bool all_active);
+
 }  // namespace fim
 
 struct fim_mesh {

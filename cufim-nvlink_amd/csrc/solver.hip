@@ -933,11 +933,11 @@ __global__ void k_conv_seed(const uint32_t *copies, uint32_t n, uint32_t inst, f
 // warm start: every copy of the instance takes the global field value,
 // everything flagged, every submesh queued
 __global__ void k_conv_init(const float *phi0, const uint32_t *c2g, uint32_t inst, float *phi,
-                            uint8_t *dirty, uint32_t ncopy) {
+                            uint8_t *dirty, uint32_t ncopy, int flag) {
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncopy; c += gridDim.x * blockDim.x) {
     const size_t gc = (size_t)inst * ncopy + c;
     phi[gc] = phi0[c2g[c]];
-    dirty[gc] = 1;
+    dirty[gc] = (uint8_t)flag;
   }
 }
 
@@ -989,6 +989,47 @@ __global__ __launch_bounds__(256) void k_conv_residual(const SubInfo *sub, uint3
     }
   }
   if (worst > 0.0f) atomicMax(res_bits, __float_as_uint(worst));
+}
+
+// support sweep: for every vertex the smallest candidate its elements give
+// (global vertex, float bits, atomicMin); seeds and unreached vertices are
+// judged by the caller
+__global__ __launch_bounds__(256) void k_conv_support(const SubInfo *sub, uint32_t nsub,
+                                                      const uint2 *e2n, const float *geo,
+                                                      const uint8_t *ereg, const float *rscale,
+                                                      const float *phi, const uint32_t *c2g,
+                                                      size_t ib, uint32_t inst, uint32_t *best) {
+  for (uint32_t s = blockIdx.x; s < nsub; s += gridDim.x) {
+    const SubInfo si = sub[s];
+    for (uint32_t e = threadIdx.x; e < si.ne; e += blockDim.x) {
+      const size_t ge = (size_t)si.eoff + e;
+      const uint2 q = e2n[ge];
+      const uint32_t nd[4] = {q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16};
+      const float sc = rscale[4 * (size_t)inst + (ereg[ge] & 3)];
+      float g[6], ph[4];
+      for (int k = 0; k < 6; ++k) g[k] = geo[6 * ge + k] * sc;
+      for (int k = 0; k < 4; ++k) ph[k] = phi[ib + si.coff + nd[k]];
+      const float c[4] = {cand_exact<0, float>(g, ph, F_INF), cand_exact<1, float>(g, ph, F_INF),
+                          cand_exact<2, float>(g, ph, F_INF), cand_exact<3, float>(g, ph, F_INF)};
+      for (int k = 0; k < 4; ++k)
+        if (c[k] < F_INF) atomicMin(&best[c2g[si.coff + nd[k]]], __float_as_uint(c[k]));
+    }
+  }
+}
+
+// worst relative amount by which a reached, non-seed vertex lies below the
+// best candidate of its elements (a field that is not a fixed point from
+// below: an initial field that was not an upper bound)
+__global__ void k_support_check(const float *phi_g, const uint32_t *best, uint32_t nn,
+                                uint32_t *worst_bits) {
+  float worst = 0.0f;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nn; v += gridDim.x * blockDim.x) {
+    const float p = phi_g[v];
+    if (!(p > 0.0f) || !(p < F_INF)) continue;
+    const float b = __uint_as_float(best[v]);
+    if (b > p) worst = fmaxf(worst, (b - p) / (1.0f + p));
+  }
+  if (worst > 0.0f) atomicMax(worst_bits, __float_as_uint(worst));
 }
 
 // ---------------------------------------------------------------------------
@@ -1444,6 +1485,37 @@ int guarded(F &&f) {
 
 }  // namespace
 
+namespace fim {
+
+void *ctx_stream(fim_ctx *c) { return c->stream; }
+int32_t ctx_device(const fim_ctx *c) { return c->dev; }
+uint32_t ctx_vertices(const fim_ctx *c) { return c->nn; }
+int32_t ctx_instances(const fim_ctx *c) { return (int32_t)c->ninst; }
+bool ctx_converge(const fim_ctx *c) { return c->o.mode == FIM_MODE_CONVERGE; }
+
+const float *ctx_gather(fim_ctx *c, int32_t inst) {
+  check_inst(c, inst);
+  k_conv_gather<<<blocks_for(c->nn, 256), 256, 0, c->stream>>>(c->d_phi, c->d_g2c_dsp, c->d_g2c, c->nn,
+                                                               (size_t)inst * c->ncopy, c->d_tmpf);
+  HIPCHECK(hipGetLastError());
+  return c->d_tmpf;
+}
+
+float *ctx_field_buffer(fim_ctx *c) { return c->d_tmpf; }
+
+void ctx_init_field(fim_ctx *c, int32_t inst, bool all_active) {
+  check_inst(c, inst);
+  k_conv_init<<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_tmpf, c->d_c2g, (uint32_t)inst,
+                                                                 c->d_phi, c->d_dirty, c->ncopy,
+                                                                 all_active ? 1 : 0);
+  if (all_active)
+    k_conv_queue_all<<<blocks_for(c->nsub, 256), 256, 0, c->stream>>>((uint32_t)inst, c->nsub, c->ninst,
+                                                                       c->d_act, c->d_wake, c->wl[0]);
+  HIPCHECK(hipGetLastError());
+}
+
+}  // namespace fim
+
 extern "C" {
 
 int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
@@ -1726,7 +1798,7 @@ int fim_init_field(fim_ctx *c, int32_t inst, const float *phi0) {
     HIPCHECK(hipSetDevice(c->dev));
     HIPCHECK(hipMemcpyAsync(c->d_tmpf, phi0, c->nn * sizeof(float), hipMemcpyHostToDevice, c->stream));
     k_conv_init<<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_tmpf, c->d_c2g, (uint32_t)inst,
-                                                                   c->d_phi, c->d_dirty, c->ncopy);
+                                                                   c->d_phi, c->d_dirty, c->ncopy, 1);
     k_conv_queue_all<<<blocks_for(c->nsub, 256), 256, 0, c->stream>>>((uint32_t)inst, c->nsub, c->ninst,
                                                                        c->d_act, c->d_wake, c->wl[0]);
     HIPCHECK(hipGetLastError());
@@ -1872,6 +1944,38 @@ int fim_residual(fim_ctx *c, int32_t inst, double *res) {
     float r;
     std::memcpy(&r, &bits, 4);
     *res = r;
+  });
+}
+
+int fim_support(fim_ctx *c, int32_t inst, double *worst) {
+  return guarded([&] {
+    if (!c || !worst) throw Error(FIM_EINVAL, "null argument");
+    if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "support check needs converge mode");
+    check_inst(c, inst);
+    HIPCHECK(hipSetDevice(c->dev));
+    uint32_t *best = nullptr;
+    HIPCHECK(hipMalloc(&best, (size_t)c->nn * sizeof(uint32_t)));
+    try {
+      k_fill<uint32_t><<<blocks_for(c->nn, 256), 256, 0, c->stream>>>(best, c->nn, 0x7f800000u);
+      k_conv_support<<<(int)std::min<uint32_t>(c->nsub, 4096u), 256, 0, c->stream>>>(
+          c->d_sub, c->nsub, c->d_e2n, c->d_geo32, c->d_ereg, c->d_rscale, c->d_phi, c->d_c2g,
+          (size_t)inst * c->ncopy, (uint32_t)inst, best);
+      k_conv_gather<<<blocks_for(c->nn, 256), 256, 0, c->stream>>>(c->d_phi, c->d_g2c_dsp, c->d_g2c, c->nn,
+                                                                   (size_t)inst * c->ncopy, c->d_tmpf);
+      k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_tmpu, 1, 0u);
+      k_support_check<<<blocks_for(c->nn, 256), 256, 0, c->stream>>>(c->d_tmpf, best, c->nn, c->d_tmpu);
+      HIPCHECK(hipGetLastError());
+      uint32_t bits = 0;
+      HIPCHECK(hipMemcpyAsync(&bits, c->d_tmpu, 4, hipMemcpyDeviceToHost, c->stream));
+      HIPCHECK(hipStreamSynchronize(c->stream));
+      float r;
+      std::memcpy(&r, &bits, 4);
+      *worst = r;
+    } catch (...) {
+      (void)hipFree(best);
+      throw;
+    }
+    HIPCHECK(hipFree(best));
   });
 }
 

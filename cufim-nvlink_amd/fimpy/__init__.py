@@ -59,6 +59,15 @@ class Opts(C.Structure):
                 ("window", C.c_double), ("unit", C.c_int32)]
 
 
+class NNOpts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("seed", C.c_uint64), ("lr", C.c_double),
+                ("beta1", C.c_double), ("beta2", C.c_double), ("adam_eps", C.c_double),
+                ("t_scale", C.c_double), ("max_maps", C.c_int32), ("max_batch", C.c_int32)]
+
+
+NN_INPUTS, NN_HIDDEN, NN_LAYERS, NN_PARAMS = 16, 256, 6, 333569
+
+
 class CfgOptions(C.Structure):
     _fields_ = [("verbose", C.c_int32), ("method_id", C.c_int32), ("solvetype", C.c_int32),
                 ("rent_en", C.c_int32), ("dt", C.c_double), ("n_steps", C.c_int64)]
@@ -102,6 +111,19 @@ SIGNATURES = {
     "fim_run": (C.c_int, [_vp, C.c_int32, _pi32]),
     "fim_download": (C.c_int, [_vp, C.c_int32, _pd, _pi32]),
     "fim_residual": (C.c_int, [_vp, C.c_int32, _pd]),
+    "fim_support": (C.c_int, [_vp, C.c_int32, _pd]),
+    "fim_nn_create": (C.c_int, [_vp, C.POINTER(NNOpts), C.POINTER(_vp)]),
+    "fim_nn_destroy": (None, [_vp]),
+    "fim_nn_get_params": (C.c_int, [_vp, _pf]),
+    "fim_nn_set_params": (C.c_int, [_vp, _pf]),
+    "fim_nn_features": (C.c_int, [_vp, _pi64, _pi64, C.c_int32, _pf, C.c_int64, _pf]),
+    "fim_nn_add_maps": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, _pi64, _pf]),
+    "fim_nn_dataset_size": (C.c_int, [_vp, _pi32]),
+    "fim_nn_gradient": (C.c_int, [_vp, _pf, _pf, C.c_int32, _pd, _pf]),
+    "fim_nn_train": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_uint64, _pd]),
+    "fim_nn_predict": (C.c_int, [_vp, _pf, C.c_int64, _pf]),
+    "fim_nn_predict_map": (C.c_int, [_vp, _pi64, C.c_int32, _pf, _pf]),
+    "fim_nn_warm_start": (C.c_int, [_vp, _vp, C.c_int32, _pi64, C.c_int32, _pf, C.c_double, C.c_int32]),
     "fim_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
     "fim_destroy": (None, [_vp]),
     "fim_device_info": (C.c_int, [C.c_int32, C.c_char_p, C.c_int64]),
@@ -289,6 +311,11 @@ class Solver:
         _check(lib().fim_residual(self.h, inst, C.byref(r)))
         return r.value
 
+    def support(self, inst=0):
+        r = C.c_double()
+        _check(lib().fim_support(self.h, inst, C.byref(r)))
+        return r.value
+
     def stats(self):
         s = Stats()
         _check(lib().fim_stats(self.h, C.byref(s)))
@@ -341,3 +368,93 @@ def run_reference(mesh, part, plan, out, reps=1, cfg="data/default.cfg", device=
                                    out.encode() if out else b"", reps, cfg.encode(), device,
                                    C.byref(s)))
     return s.value
+
+
+def _sites3(sites):
+    s = np.full(3, -1, dtype=np.int64)
+    v = np.asarray(sites, dtype=np.int64).reshape(-1)
+    assert 1 <= v.shape[0] <= 3
+    s[:v.shape[0]] = v
+    return s, int(v.shape[0])
+
+
+class Surrogate:
+    """Coordinate MLP surrogate (include/cufim.h, fim_nn_*)."""
+
+    def __init__(self, mesh: Mesh, device=0, seed=0, lr=0.0, t_scale=0.0, max_maps=0, max_batch=0):
+        self.mesh = mesh
+        self.nn = mesh.nn
+        o = NNOpts(device, seed, lr, 0.0, 0.0, 0.0, t_scale, max_maps, max_batch)
+        self.h = _vp()
+        _check(lib().fim_nn_create(mesh.h, C.byref(o), C.byref(self.h)))
+
+    def params(self):
+        p = np.zeros(NN_PARAMS, dtype=np.float32)
+        _check(lib().fim_nn_get_params(self.h, _p(p, _pf)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (NN_PARAMS,)
+        _check(lib().fim_nn_set_params(self.h, _p(p, _pf)))
+
+    def features(self, vtx, sites, scales):
+        v = np.ascontiguousarray(vtx, dtype=np.int64).reshape(-1)
+        s, ns = _sites3(sites)
+        sc = np.ascontiguousarray(scales, dtype=np.float32)
+        x = np.zeros((v.shape[0], NN_INPUTS), dtype=np.float32)
+        _check(lib().fim_nn_features(self.h, _p(v, _pi64), _p(s, _pi64), ns, _p(sc, _pf), v.shape[0],
+                                     _p(x, _pf)))
+        return x
+
+    def add_maps(self, solver, inst0, sites, scales):
+        """sites: list of 1-3 vertex ids per map; scales: (n, 4)"""
+        n = len(sites)
+        s = np.stack([_sites3(x)[0] for x in sites]).astype(np.int64)
+        sc = np.ascontiguousarray(scales, dtype=np.float32).reshape(n, 4)
+        _check(lib().fim_nn_add_maps(self.h, solver.h, inst0, n, _p(s, _pi64), _p(sc, _pf)))
+
+    def dataset_size(self):
+        n = C.c_int32()
+        _check(lib().fim_nn_dataset_size(self.h, C.byref(n)))
+        return n.value
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(NN_PARAMS, dtype=np.float32)
+        loss = C.c_double()
+        _check(lib().fim_nn_gradient(self.h, _p(x, _pf), _p(t, _pf), x.shape[0], C.byref(loss), _p(g, _pf)))
+        return loss.value, g
+
+    def train(self, steps, batch=65536, seed=0):
+        loss = C.c_double()
+        _check(lib().fim_nn_train(self.h, steps, batch, seed, C.byref(loss)))
+        return loss.value
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.zeros(x.shape[0], dtype=np.float32)
+        _check(lib().fim_nn_predict(self.h, _p(x, _pf), x.shape[0], _p(t, _pf)))
+        return t
+
+    def predict_map(self, sites, scales):
+        s, ns = _sites3(sites)
+        sc = np.ascontiguousarray(scales, dtype=np.float32)
+        t = np.zeros(self.nn, dtype=np.float32)
+        _check(lib().fim_nn_predict_map(self.h, _p(s, _pi64), ns, _p(sc, _pf), _p(t, _pf)))
+        return t
+
+    def warm_start(self, solver, inst, sites, scales, delta, all_active=True):
+        s, ns = _sites3(sites)
+        sc = np.ascontiguousarray(scales, dtype=np.float32)
+        _check(lib().fim_nn_warm_start(self.h, solver.h, inst, _p(s, _pi64), ns, _p(sc, _pf), delta,
+                                       int(all_active)))
+
+    def close(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.fim_nn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

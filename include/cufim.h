@@ -231,8 +231,77 @@ int fim_download(fim_ctx *c, int32_t inst, double *phi, int32_t *state);
 /* FIM residual sweep: max over vertices of (phi - update(phi)) / (1 + phi)
  * for one instance (converge mode), the fixed-point test */
 int fim_residual(fim_ctx *c, int32_t inst, double *res);
+/* Support check (no counterpart in the reference): max over reached,
+ * non-seed vertices (0 < phi < inf) of (best - phi) / (1 + phi), where best is
+ * the smallest candidate the vertex's elements give.  A field reached from an
+ * upper bound satisfies best <= phi up to the convergence threshold; a
+ * larger value means some vertex started below its fixed-point time (a warm
+ * start that was not an upper bound) and the map must be recomputed. */
+int fim_support(fim_ctx *c, int32_t inst, double *worst);
 int fim_stats(const fim_ctx *c, fim_stats_t *s);
 void fim_destroy(fim_ctx *c);
+
+/* ========================================================================
+ * Surrogate warm start (no counterpart in the reference)
+ *
+ * A coordinate MLP: 16 inputs -- vertex xyz, three pacing-site xyz (fewer
+ * sites repeat the first), four regional speed factors -- 6 hidden layers of
+ * 256 with SiLU, one output (activation time / t_scale).  BF16 weights and
+ * activations on MFMA with FP32 accumulation, FP32 master weights, Adam.
+ * Coordinates are normalised by the bounding box of the mesh given at
+ * creation.  Training maps come from a converge-mode context on the same
+ * mesh (device to device); a prediction shifted up by delta initialises an
+ * instance as an upper bound.
+ * ===================================================================== */
+typedef struct fim_nn fim_nn;
+
+typedef struct {
+  int32_t device;
+  uint64_t seed;         /* weight initialisation */
+  double lr;             /* Adam step (0 = 1e-3) */
+  double beta1, beta2;   /* Adam moments (0 = 0.9, 0.999) */
+  double adam_eps;       /* (0 = 1e-8) */
+  double t_scale;        /* us per output unit (0 = 1e5) */
+  int32_t max_maps;      /* dataset capacity in maps (0 = 4096) */
+  int32_t max_batch;     /* largest training batch (0 = 65536, multiple of 64) */
+} fim_nn_opts;
+
+#define FIM_NN_INPUTS 16
+#define FIM_NN_HIDDEN 256
+#define FIM_NN_LAYERS 6
+/* parameters: W0[256x16] b0[256] (W_l[256x256] b_l[256]) x 5, w_out[256], b_out */
+#define FIM_NN_PARAMS 333569
+
+int fim_nn_create(const fim_mesh *m, const fim_nn_opts *o, fim_nn **out);
+void fim_nn_destroy(fim_nn *nn);
+/* FP32 master parameters in the order above (FIM_NN_PARAMS values) */
+int fim_nn_get_params(fim_nn *nn, float *params);
+int fim_nn_set_params(fim_nn *nn, const float *params);
+/* inputs of one solve: sites (n <= 3 global vertex ids), scales[4] */
+int fim_nn_features(fim_nn *nn, const int64_t *vtx, const int64_t *sites, int32_t n_sites,
+                    const float *scales, int64_t n, float *x);
+/* dataset: append maps of instances [inst0, inst0+n) of a converge-mode
+ * context on the creation mesh; sites: n x 3 (ids, -1 pads), scales: n x 4 */
+int fim_nn_add_maps(fim_nn *nn, fim_ctx *c, int32_t inst0, int32_t n, const int64_t *sites,
+                    const float *scales);
+int fim_nn_dataset_size(const fim_nn *nn, int32_t *maps);
+/* one forward + backward on given inputs (x: b x 16, t: b targets in us),
+ * b a multiple of 64; returns the mean squared error (of t / t_scale) and
+ * the gradient (FIM_NN_PARAMS values); the parameters do not change */
+int fim_nn_gradient(fim_nn *nn, const float *x, const float *t, int32_t b, double *loss,
+                    float *grad);
+/* Adam training on random (map, vertex) samples of the dataset */
+int fim_nn_train(fim_nn *nn, int32_t steps, int32_t batch, uint64_t seed, double *loss);
+/* prediction (us) for n inputs x (n x 16) */
+int fim_nn_predict(fim_nn *nn, const float *x, int64_t n, float *t);
+/* prediction for every vertex of the creation mesh for one solve (us) */
+int fim_nn_predict_map(fim_nn *nn, const int64_t *sites, int32_t n_sites, const float *scales,
+                       float *t);
+/* warm start of instance inst of c: field = prediction + delta (us),
+ * sites = 0; all_active marks every vertex active (else only the times are
+ * set, and the caller seeds the sites) */
+int fim_nn_warm_start(fim_nn *nn, fim_ctx *c, int32_t inst, const int64_t *sites,
+                      int32_t n_sites, const float *scales, double delta, int32_t all_active);
 
 /* ========================================================================
  * Reference-format configuration and end-to-end run
