@@ -1,0 +1,140 @@
+"""Surrogate warm start: training maps from the batched solver, training of
+the coordinate MLP, calibration of the upward shift delta, and warm-started
+batched solves judged by the support check with a cold re-solve as fallback.
+
+A solve is (sites, scales): 1-3 pacing vertices (time 0) and four regional
+speed factors.  All work runs through libcufim.so (fim_nn_*, fim_*).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import PART_RCB, Pack, Solver, Surrogate  # noqa: F401
+from .workloads import pacing_sites, splitmix64
+
+
+def draw_solves(cfg, n, seed, max_sites=3):
+    """n solves on a box config: 1..max_sites pacing vertices of the z = 0
+    face and four factors uniform in [0.5, 1.5], from SplitMix64(seed)."""
+    g = splitmix64(seed)
+    out = []
+    for _ in range(n):
+        k = 1 + next(g) % max_sites
+        sites = pacing_sites(cfg, k, next(g))
+        sc = np.array([0.5 + (next(g) >> 11) * 2.0 ** -53 for _ in range(4)], dtype=np.float32)
+        out.append((sites, sc))
+    return out
+
+
+def _chunks(seq, n):
+    for i in range(0, len(seq), n):
+        yield i, seq[i:i + n]
+
+
+def cold_batch(solver, solves):
+    """Cold batched solve of len(solves) instances; returns the solver stats."""
+    solver.reset()
+    for i, (sites, sc) in enumerate(solves):
+        solver.set_scaling(sc, inst=i)
+        solver.seed(sites, 0.0, inst=i)
+    solver.run(max_iters=1_000_000)
+    return solver.stats()
+
+
+def _note(on_stats, st):
+    if on_stats is not None:
+        on_stats(st)
+
+
+def build_dataset(nn: Surrogate, pack: Pack, solves, batch=256, window=2000.0, unit=0,
+                  profile=False, on_stats=None):
+    """Solves every solve cold in batches and appends the maps to the
+    surrogate's dataset.  Returns seconds spent.  on_stats(stats) is called
+    after every batch (profile=True times every compute launch)."""
+    t0 = time.perf_counter()
+    s = Solver(pack, n_instances=batch, window=window, unit=unit, profile=profile)
+    try:
+        for i, chunk in _chunks(solves, batch):
+            _note(on_stats, cold_batch(s, chunk + [chunk[0]] * (batch - len(chunk))))
+            nn.add_maps(s, 0, [c[0] for c in chunk], np.stack([c[1] for c in chunk]))
+    finally:
+        s.close()
+    return time.perf_counter() - t0
+
+
+def surrogate_errors(nn: Surrogate, pack: Pack, solves, batch=256, window=2000.0, profile=False,
+                     on_stats=None):
+    """Per-solve (max underestimate, max |error|, mean |error|, max T) of the
+    bare surrogate against the converged FIM maps, in us."""
+    out = []
+    s = Solver(pack, n_instances=batch, window=window, profile=profile)
+    try:
+        for i, chunk in _chunks(solves, batch):
+            _note(on_stats, cold_batch(s, chunk + [chunk[0]] * (batch - len(chunk))))
+            for j, (sites, sc) in enumerate(chunk):
+                ref, _ = s.download(inst=j)
+                pred = nn.predict_map(sites, sc)
+                d = ref - pred
+                out.append((float(d.max()), float(np.abs(d).max()), float(np.abs(d).mean()),
+                            float(ref.max())))
+    finally:
+        s.close()
+    return out
+
+
+def warm_batch(nn: Surrogate, solver: Solver, solves, delta, all_active=True):
+    """Warm-started batch: prediction + delta as the initial field of every
+    instance, then FIM to the fixed point.  Returns (stats, seconds of
+    inference + initialisation, seconds of FIM)."""
+    solver.reset()
+    t0 = time.perf_counter()
+    for i, (sites, sc) in enumerate(solves):
+        solver.set_scaling(sc, inst=i)
+        nn.warm_start(solver, i, sites, sc, delta, all_active=all_active)
+        if not all_active:
+            solver.seed(sites, 0.0, inst=i)
+    t1 = time.perf_counter()
+    solver.run(max_iters=1_000_000)
+    t2 = time.perf_counter()
+    return solver.stats(), t1 - t0, t2 - t1
+
+
+def judge(solver: Solver, n, tol=1e-4):
+    """Instances whose field lies below its fixed point (support > tol)."""
+    return [i for i in range(n) if solver.support(inst=i) > tol]
+
+
+def warm_solve_all(nn: Surrogate, pack: Pack, solves, delta, batch=256, window=2000.0, unit=0,
+                   all_active=True, tol=1e-4, judge_maps=True, profile=False, on_stats=None):
+    """Warm-started batched solves with the support check as judge and a cold
+    re-solve of the rejected ones.  Returns a dict of times and counts."""
+    s = Solver(pack, n_instances=batch, window=window, unit=unit, profile=profile)
+    fb = Solver(pack, n_instances=batch, window=window, unit=unit, profile=profile)
+    rep = dict(maps=0, infer_s=0.0, fim_s=0.0, judge_s=0.0, fallback_s=0.0, fallbacks=0,
+               global_iters=0, activations=0, elem_updates=0)
+    try:
+        for i, chunk in _chunks(solves, batch):
+            full = chunk + [chunk[0]] * (batch - len(chunk))
+            st, ti, tf = warm_batch(nn, s, full, delta, all_active)
+            _note(on_stats, st)
+            rep["infer_s"] += ti
+            rep["fim_s"] += tf
+            for k in ("global_iters", "activations", "elem_updates"):
+                rep[k] += st[k]
+            if judge_maps:
+                t0 = time.perf_counter()
+                bad = judge(s, len(chunk), tol)
+                rep["judge_s"] += time.perf_counter() - t0
+                if bad:
+                    t0 = time.perf_counter()
+                    redo = [chunk[j] for j in bad]
+                    _note(on_stats, cold_batch(fb, redo + [redo[0]] * (batch - len(redo))))
+                    rep["fallback_s"] += time.perf_counter() - t0
+                    rep["fallbacks"] += len(bad)
+            rep["maps"] += len(chunk)
+    finally:
+        s.close()
+        fb.close()
+    return rep
