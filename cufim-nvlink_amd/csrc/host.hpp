@@ -54,6 +54,8 @@ void save_vtk(const Mesh &m, const std::string &path, const double *phi,
 std::vector<int32_t> read_tags(const std::string &path);
 Mesh gen_box(double lx, double ly, double lz, int nx, int ny, int nz, double jitter,
              uint64_t seed, double a0, double a1, double vf, double vc);
+Mesh gen_shell(double ao, double bo, double ai, double bi, double h, int n_scars, double r0,
+               double r1, uint64_t seed, double vf, double vc, double vs);
 
 // ---------------------------------------------------------------------------
 // Partition: elements grouped into submeshes, vertices duplicated per

@@ -144,6 +144,23 @@ int fim_mesh_save_vtk(const fim_mesh *m, const char *path, const double *phi, co
 
 void fim_mesh_destroy(fim_mesh *m) { delete m; }
 
+int fim_mesh_gen_shell(double ao, double bo, double ai, double bi, double h, int32_t n_scars,
+                       double r0, double r1, uint64_t seed, double vf, double vc, double vs,
+                       fim_mesh **out) {
+  return guarded([&] {
+    if (!out) throw Error(FIM_EINVAL, "null argument");
+    *out = nullptr;
+    fim_mesh *m = new fim_mesh();
+    try {
+      m->m = gen_shell(ao, bo, ai, bi, h, n_scars, r0, r1, seed, vf, vc, vs);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
 int fim_mesh_gen_box(double lx, double ly, double lz, int nx, int ny, int nz, double jitter,
                      uint64_t seed, double angle0, double angle1, double vf, double vc,
                      fim_mesh **out) {

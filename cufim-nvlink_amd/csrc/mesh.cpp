@@ -295,4 +295,123 @@ Mesh gen_box(double lx, double ly, double lz, int nx, int ny, int nz, double jit
   return m;
 }
 
+// Prolate-ellipsoid shell: hexes of side h whose centre lies between the
+// inner and outer ellipsoid (semi-axes ai/bi/bi and ao/bo/bo along x, y, z,
+// centred at the origin), each split into the same 6 tets as gen_box
+// (conforming).  Fibres rotate from +60 deg (inner surface) to -60 deg
+// (outer surface) in the tangent plane, from the circumferential direction
+// about the long axis towards the longitudinal one; speeds vf / vc, and
+// vs inside each of n_scars seeded spheres (radius r0..r1, centres at
+// random kept hex centres).  Region id 1 marks scar elements.
+Mesh gen_shell(double ao, double bo, double ai, double bi, double h, int n_scars, double r0,
+               double r1, uint64_t seed, double vf, double vc, double vs) {
+  if (!(h > 0) || !(ai > 0) || !(bi > 0) || !(ao > ai) || !(bo > bi))
+    throw Error(FIM_EINVAL, "bad shell dimensions");
+  if (n_scars < 0 || !(r1 >= r0) || !(r0 >= 0)) throw Error(FIM_EINVAL, "bad scar parameters");
+  const int64_t nx = (int64_t)std::ceil(2 * ao / h), ny = (int64_t)std::ceil(2 * bo / h);
+  const int64_t nz = ny;
+  if (nx * ny * nz > (int64_t)4e9) throw Error(FIM_ELIMIT, "shell grid too large");
+  const double x0 = -0.5 * nx * h, y0 = -0.5 * ny * h, z0 = -0.5 * nz * h;
+  auto rr = [](double x, double y, double z, double a, double b) {
+    return std::sqrt((x / a) * (x / a) + (y / b) * (y / b) + (z / b) * (z / b));
+  };
+  auto inside = [&](double x, double y, double z) {
+    return rr(x, y, z, ao, bo) <= 1.0 && rr(x, y, z, ai, bi) >= 1.0;
+  };
+  // kept cells and used grid vertices
+  const int64_t NX = nx + 1, NY = ny + 1;
+  std::vector<int64_t> cells;
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i)
+        if (inside(x0 + (i + 0.5) * h, y0 + (j + 0.5) * h, z0 + (k + 0.5) * h))
+          cells.push_back(i + nx * (j + ny * k));
+  if (cells.empty()) throw Error(FIM_EINVAL, "shell has no cells at this spacing");
+  std::vector<int64_t> vid;  // grid vertex -> mesh vertex (sorted map via sort/unique)
+  std::vector<int64_t> used;
+  used.reserve(cells.size() * 2);
+  for (int64_t c : cells) {
+    const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    for (int dk = 0; dk < 2; ++dk)
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) used.push_back((i + di) + NX * ((j + dj) + NY * (k + dk)));
+  }
+  std::sort(used.begin(), used.end());
+  used.erase(std::unique(used.begin(), used.end()), used.end());
+  auto vindex = [&](int64_t g) { return (int64_t)(std::lower_bound(used.begin(), used.end(), g) - used.begin()); };
+  Mesh m;
+  m.nn = (int64_t)used.size();
+  m.ne = 6 * (int64_t)cells.size();
+  m.xyz.resize(3 * m.nn);
+  for (int64_t v = 0; v < m.nn; ++v) {
+    const int64_t g = used[v];
+    m.xyz[3 * v] = x0 + (g % NX) * h;
+    m.xyz[3 * v + 1] = y0 + ((g / NX) % NY) * h;
+    m.xyz[3 * v + 2] = z0 + (g / (NX * NY)) * h;
+  }
+  // scars
+  Xoshiro256ss rng(seed);
+  std::vector<double> sc(4 * (size_t)n_scars);
+  for (int s = 0; s < n_scars; ++s) {
+    const int64_t c = cells[(size_t)(rng.uniform() * cells.size()) % cells.size()];
+    sc[4 * s] = x0 + (c % nx + 0.5) * h;
+    sc[4 * s + 1] = y0 + ((c / nx) % ny + 0.5) * h;
+    sc[4 * s + 2] = z0 + (c / (nx * ny) + 0.5) * h;
+    sc[4 * s + 3] = r0 + (r1 - r0) * rng.uniform();
+  }
+  static const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  m.e2n.resize(4 * m.ne);
+  m.fib.resize(3 * m.ne);
+  m.vel.resize(2 * m.ne);
+  m.region.assign(m.ne, 0);
+  int64_t e = 0;
+  for (int64_t cc : cells) {
+    const int64_t ci = cc % nx, cj = (cc / nx) % ny, ck = cc / (nx * ny);
+    const double x = x0 + (ci + 0.5) * h, y = y0 + (cj + 0.5) * h, z = z0 + (ck + 0.5) * h;
+    // transmural depth: 0 on the inner surface, 1 on the outer
+    const double rin = rr(x, y, z, ai, bi), rout = rr(x, y, z, ao, bo);
+    const double dep = std::min(1.0, std::max(0.0, (rin - 1.0) / std::max(rin - rout, 1e-12)));
+    const double th = (60.0 - 120.0 * dep) * (M_PI / 180.0);
+    double n[3] = {x / (ao * ao), y / (bo * bo), z / (bo * bo)};
+    double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (double &q : n) q /= nl;
+    double cr[3] = {0.0, -z, y};
+    double cl = std::sqrt(y * y + z * z);
+    if (cl < 1e-9) {
+      cr[1] = 1.0;
+      cr[2] = 0.0;
+      cl = 1.0;
+    }
+    for (double &q : cr) q /= cl;
+    // longitudinal tangent = n x c
+    double lo[3] = {n[1] * cr[2] - n[2] * cr[1], n[2] * cr[0] - n[0] * cr[2], n[0] * cr[1] - n[1] * cr[0]};
+    const double ll = std::sqrt(lo[0] * lo[0] + lo[1] * lo[1] + lo[2] * lo[2]);
+    for (double &q : lo) q /= ll;
+    double f[3];
+    for (int d = 0; d < 3; ++d) f[d] = std::cos(th) * cr[d] + std::sin(th) * lo[d];
+    bool scar = false;
+    for (int s = 0; s < n_scars && !scar; ++s) {
+      const double dx = x - sc[4 * s], dy = y - sc[4 * s + 1], dz = z - sc[4 * s + 2];
+      scar = dx * dx + dy * dy + dz * dz <= sc[4 * s + 3] * sc[4 * s + 3];
+    }
+    for (int p = 0; p < 6; ++p) {
+      int64_t c[3] = {ci, cj, ck};
+      int64_t *out = &m.e2n[4 * e];
+      out[0] = vindex(c[0] + NX * (c[1] + NY * c[2]));
+      for (int s = 0; s < 3; ++s) {
+        c[perm[p][s]]++;
+        out[s + 1] = vindex(c[0] + NX * (c[1] + NY * c[2]));
+      }
+      for (int d = 0; d < 3; ++d) m.fib[3 * e + d] = f[d];
+      m.vel[2 * e] = scar ? vs : vf;
+      m.vel[2 * e + 1] = scar ? vs : vc;
+      m.region[e] = scar ? 1 : 0;
+      ++e;
+    }
+  }
+  m.etags.assign(m.ne, 1);
+  m.finalize();
+  return m;
+}
+
 }  // namespace fim

@@ -98,6 +98,9 @@ SIGNATURES = {
                                    C.c_double, C.c_uint64, C.c_double, C.c_double, C.c_double,
                                    C.c_double, C.POINTER(_vp)]),
     "fim_mesh_concat": (C.c_int, [C.POINTER(_vp), C.c_int32, C.POINTER(_vp)]),
+    "fim_mesh_gen_shell": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_int32, C.c_double, C.c_double, C.c_uint64, C.c_double,
+                                     C.c_double, C.c_double, C.POINTER(_vp)]),
     "fim_pack_build": (C.c_int, [_vp, C.POINTER(PartOpts), C.POINTER(_vp)]),
     "fim_pack_info": (C.c_int, [_vp, C.POINTER(PackInfo)]),
     "fim_pack_partition": (C.c_int, [_vp, _pi64, _pi64]),
@@ -192,6 +195,13 @@ class Mesh:
         _check(lib().fim_mesh_gen_box(lx, ly, lz, nx, ny, nz, jitter, seed, angle0, angle1, vf, vc,
                                       C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def gen_shell(cls, ao, bo, ai, bi, h, n_scars=12, r0=3000.0, r1=10000.0, seed=0, vf=0.6, vc=0.24,
+                  vs=0.05):
+        hh = _vp()
+        _check(lib().fim_mesh_gen_shell(ao, bo, ai, bi, h, n_scars, r0, r1, seed, vf, vc, vs, C.byref(hh)))
+        return cls(hh)
 
     @classmethod
     def concat(cls, parts):

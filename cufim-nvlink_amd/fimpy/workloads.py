@@ -112,3 +112,29 @@ def multi_solves(parts, offs, n, seed):
         scale = np.array([0.5 + (next(g) >> 11) * 2.0 ** -53 for _ in range(4)], dtype=np.float32)
         out.append(dict(part=k, sites=sites, scale=scale))
     return out
+
+
+# ---------------------------------------------------------------------------
+# prolate-ellipsoid shell, semi-axes 5/3/3 cm outer and 4.2/2.2/2.2 cm inner,
+# hexes of 395 um -> ~10M tets; twelve scars of radius 0.3-1.0 cm with
+# 0.05 m/s; fibres +60 -> -60 deg transmurally; v_f 0.6 / v_c 0.24 m/s
+# ---------------------------------------------------------------------------
+SHELL = dict(ao=50000.0, bo=30000.0, ai=42000.0, bi=22000.0, h=395.0, n_scars=12, r0=3000.0,
+             r1=10000.0, vf=0.6, vc=0.24, vs=0.05)
+
+
+def shell(seed, h=None) -> Mesh:
+    cfg = dict(SHELL)
+    if h is not None:
+        cfg["h"] = h
+    return Mesh.gen_shell(seed=seed, **cfg)
+
+
+def inner_surface_sites(mesh: Mesh, k, seed, cfg=None):
+    """k distinct vertices within 1.5 hex sides of the inner ellipsoid."""
+    cfg = cfg or SHELL
+    xyz, e2n, _ = mesh.arrays()
+    h = float(np.abs(xyz[e2n[0, 1]] - xyz[e2n[0, 0]]).max())
+    r = np.sqrt((xyz[:, 0] / cfg["ai"]) ** 2 + (xyz[:, 1] ** 2 + xyz[:, 2] ** 2) / cfg["bi"] ** 2)
+    cand = np.flatnonzero(r <= 1.0 + 1.5 * h / cfg["bi"])
+    return cand[pick(cand.shape[0], k, seed)]

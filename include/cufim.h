@@ -116,6 +116,17 @@ int fim_mesh_gen_box(double lx, double ly, double lz, int nx, int ny, int nz,
                      double jitter, uint64_t seed, double angle0, double angle1,
                      double vf, double vc, fim_mesh **out);
 
+/* Prolate-ellipsoid shell: hexes of side h (um) whose centre lies between
+ * the inner (semi-axes ai, bi, bi) and outer (ao, bo, bo) ellipsoids about the
+ * origin, 6 tets each (same split as the box).  Fibres rotate transmurally
+ * from +60 deg (inner) to -60 deg (outer) in the tangent plane, from the
+ * circumferential direction about the x axis towards the longitudinal one;
+ * speeds vf / vc, and vs (both) inside n_scars seeded spheres of radius
+ * r0..r1 centred at random kept hexes.  Region id 1 marks scar elements. */
+int fim_mesh_gen_shell(double ao, double bo, double ai, double bi, double h, int32_t n_scars,
+                       double r0, double r1, uint64_t seed, double vf, double vc, double vs,
+                       fim_mesh **out);
+
 /* Disjoint union of meshes (vertices and elements renumbered by offsets, in
  * argument order; tags, fibres, speeds and regions carried over).  One
  * context on the union batches solves of different meshes: an instance

@@ -260,3 +260,18 @@ def test_multi_mesh_batch_with_regional_scalings():
         reached = np.flatnonzero(ref < 1e50)
         assert reached.min() >= bounds[k] and reached.max() < bounds[k + 1]
     s.close()
+
+
+def test_scarred_shell_vs_fp64_oracle():
+    """prolate shell with twelve slow scars at the 1M-tet resolution the
+    large workload is checked at, three pacing sites on the inner surface"""
+    from fimpy import workloads as W
+    m = W.shell(1, h=850.0)
+    xyz, e2n, _ = m.arrays()
+    fib, vel = m.material()
+    om = O.Mesh(xyz, e2n, fib=fib, vel=vel)
+    p = F.Pack(m, F.PART_RCB, max_nn=1024, seed=1)
+    sites = W.inner_surface_sites(m, 3, 5)
+    phi, st, it, res, stats = _converge(m, p, sites, window=2000.0)
+    ref, _ = O.fim_solve(om, sites, variant=O.LOCAL_EXACT)
+    _check_fp32(phi, ref)

@@ -269,3 +269,37 @@ def test_multi_mesh_workload_is_seeded():
         k = s["part"]
         assert np.all((s["sites"] >= bounds[k]) & (s["sites"] < bounds[k + 1]))
         assert np.all((s["scale"] >= 0.5) & (s["scale"] <= 1.5))
+
+
+def test_shell_generator_is_seeded_conforming_and_scarred():
+    from fimpy import workloads as W
+    a = W.shell(1, h=2500.0)
+    b = W.shell(1, h=2500.0)
+    c = W.shell(2, h=2500.0)
+    xa, ea, _ = a.arrays()
+    xb, eb, _ = b.arrays()
+    assert np.array_equal(xa, xb) and np.array_equal(ea, eb)
+    fa, va = a.material()
+    fc, vc = c.material()
+    assert not np.array_equal(va, vc)  # scars move with the seed
+    assert np.allclose(np.linalg.norm(fa, axis=1), 1.0)
+    scar = va[:, 0] < 0.1
+    assert 0.01 < scar.mean() < 0.5 and np.all(va[scar] == 0.05)
+    assert np.all(va[~scar, 0] == 0.6) and np.all(va[~scar, 1] == 0.24)
+    # element centres lie in the shell
+    cen = xa[ea].mean(axis=1)
+    s = W.SHELL
+    ro = np.sqrt((cen[:, 0] / s["ao"]) ** 2 + (cen[:, 1] ** 2 + cen[:, 2] ** 2) / s["bo"] ** 2)
+    ri = np.sqrt((cen[:, 0] / s["ai"]) ** 2 + (cen[:, 1] ** 2 + cen[:, 2] ** 2) / s["bi"] ** 2)
+    assert ro.max() <= 1.0 + 1e-9 + 2500.0 / s["bo"] and ri.min() >= 1.0 - 2500.0 / s["bi"]
+    # conforming: no face shared by more than two elements
+    faces = np.sort(np.concatenate([ea[:, [1, 2, 3]], ea[:, [0, 2, 3]], ea[:, [0, 1, 3]], ea[:, [0, 1, 2]]]), axis=1)
+    _, cnt = np.unique(faces, axis=0, return_counts=True)
+    assert cnt.max() == 2
+    # fibres tangent to the ellipsoidal shells (orthogonal to the normal,
+    # which the generator evaluates at the hex centre, not the tet centroid)
+    n = np.stack([cen[:, 0] / s["ao"] ** 2, cen[:, 1] / s["bo"] ** 2, cen[:, 2] / s["bo"] ** 2], 1)
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    assert np.abs(np.einsum("ij,ij->i", n, fa)).max() < 0.1  # normal taken at the hex centre
+    sites = W.inner_surface_sites(a, 3, 7)
+    assert len(set(sites.tolist())) == 3
