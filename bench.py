@@ -70,6 +70,7 @@ def traffic_from_profiles(args):
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
     c = re.search(r"traffic_config:\s*(.*)", text)
     mine = (f"batch={args.batch} max_nn={args.max_nn} block={args.block} unit={args.unit} "
+            f"local_iters={args.local_iters} "
             f"window={float(args.window)} staged={args.staged} nn={not args.no_nn}")
     src = re.search(r"traffic_source:\s*([0-9a-f]+)", text)
     if not m or not c or c.group(1).strip() != mine or not src or src.group(1) != kernel_source_digest():
@@ -100,7 +101,8 @@ def nn_leg(args, account, plain_s):
     train = WS.draw_solves(cfg, args.nn_maps, 1001 + args.seed)
     val = WS.draw_solves(cfg, 64, 2002 + args.seed)
     test = WS.draw_solves(cfg, args.nn_test, 3003 + args.seed)
-    kw = dict(batch=args.batch, window=args.window, profile=True, on_stats=account)
+    kw = dict(batch=args.batch, window=args.window, profile=True, on_stats=account,
+              local_iters=args.local_iters)
     nn = F.Surrogate(mesh, seed=args.seed, max_maps=args.nn_maps, max_batch=args.nn_batch)
     t_data = WS.build_dataset(nn, pack, train, unit=args.unit, **kw)
     log(f"[bench] surrogate dataset: {args.nn_maps} maps in {t_data:.2f}s")
@@ -112,7 +114,8 @@ def nn_leg(args, account, plain_s):
     under = max(e[0] for e in errs)
     delta = 1.25 * max(under, 0.0)
     # held-out solves: cold batched (timed), surrogate error on each map
-    s = F.Solver(pack, n_instances=args.batch, window=args.window, unit=args.unit, profile=True)
+    s = F.Solver(pack, n_instances=args.batch, window=args.window, unit=args.unit, profile=True,
+                 local_iters=args.local_iters)
     t_cold, cold_it, cold_ev, sur = 0.0, 0, 0, []
     for i in range(0, len(test), args.batch):
         chunk = test[i:i + args.batch]
@@ -185,6 +188,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--max-nn", type=int, default=1024)
     ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--local-iters", type=int, default=4,
+                    help="sweeps per activation before requeueing (0 = to local convergence)")
     ap.add_argument("--unit", type=int, default=2,
                     help="queued instances of one submesh run back to back by one workgroup")
     ap.add_argument("--seed", type=int, default=2)
@@ -198,7 +203,8 @@ def main():
     # plain single-solve GPU FIM: its own best setting (fewer, smaller work
     # items want smaller submeshes and a finer window)
     ap.add_argument("--plain-max-nn", type=int, default=256)
-    ap.add_argument("--plain-window", type=float, default=1000.0)
+    ap.add_argument("--plain-window", type=float, default=2000.0)
+    ap.add_argument("--plain-local-iters", type=int, default=4)
     ap.add_argument("--no-nn", action="store_true", help="skip the surrogate warm-start leg")
     ap.add_argument("--nn-maps", type=int, default=4096)
     ap.add_argument("--nn-steps", type=int, default=4000)
@@ -236,6 +242,7 @@ def main():
 
     # ---- batched solver, timed -----------------------------------------
     s = F.Solver(pack, n_instances=args.batch, block=args.block, profile=True, unit=args.unit,
+                 local_iters=args.local_iters,
                  window=args.window, staged=args.staged)
     for i in range(args.warmup):
         solve_batch(s, sets)
@@ -263,6 +270,7 @@ def main():
     graph_maps_per_s = None
     if args.graph_steps > 0:
         g = F.Solver(pack, n_instances=args.batch, block=args.block, window=args.window, unit=args.unit,
+                     local_iters=args.local_iters,
                      staged=args.staged)
         solve_batch(g, sets)
         t0 = time.perf_counter()
@@ -277,6 +285,7 @@ def main():
     ppack = pack if args.plain_max_nn == args.max_nn else \
         F.Pack(mesh, F.PART_RCB, max_nn=args.plain_max_nn, seed=args.seed)
     p1 = F.Solver(ppack, n_instances=1, block=args.block, profile=True, window=args.plain_window,
+                  local_iters=args.plain_local_iters,
                   staged=args.staged)
     solve_batch(p1, sets[:1])
     account(p1.stats())
@@ -340,6 +349,7 @@ def main():
                          f"{info['nsub']} submeshes",
             "workgroup": args.block,
             "unit": args.unit,
+            "local_iters": args.local_iters,
             "window_us": args.window,
             "staged": args.staged,
         },
@@ -349,7 +359,8 @@ def main():
         "residual_inst0": res0,
         "graph_maps_per_s": graph_maps_per_s,
         "plain_maps_per_s": 1.0 / plain_s,
-        "plain_config": {"max_nn": args.plain_max_nn, "window_us": args.plain_window},
+        "plain_config": {"max_nn": args.plain_max_nn, "window_us": args.plain_window,
+                         "local_iters": args.plain_local_iters},
         "batched_over_plain": maps_per_s * plain_s,
         "solves_per_s": maps_per_s,
         "elem_evals_per_s": tot["elem_updates"] / elapsed,

@@ -28,7 +28,7 @@ SPACE = {
     "local_iters": [0, 1, 2, 4, 8],
     "window": [500.0, 1000.0, 2000.0, 4000.0],
 }
-DEFAULT = {"max_nn": 1024, "block": 256, "local_iters": 0, "window": 2000.0}
+DEFAULT = {"max_nn": 1024, "block": 256, "local_iters": 4, "window": 2000.0}
 
 
 def signature(mesh: Mesh, n_instances: int) -> str:
