@@ -2,6 +2,7 @@
 // the thread-local error message.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 
@@ -275,6 +276,38 @@ int fim_pack_partition(const fim_pack *p, int64_t *e2s, int64_t *c2g) {
 }
 
 void fim_pack_destroy(fim_pack *p) { delete p; }
+
+int fim_pack_save_tags(const fim_pack *p, const char *path) {
+  return guarded([&] {
+    if (!p || !path) throw Error(FIM_EINVAL, "null argument");
+    FILE *f = std::fopen(path, "w");
+    if (!f) throw Error(FIM_EIO, std::string("cannot write ") + path);
+    for (int64_t s : p->p.part.e2s) std::fprintf(f, "%lld\n", (long long)s);
+    if (std::fclose(f) != 0) throw Error(FIM_EIO, std::string("cannot write ") + path);
+  });
+}
+
+int fim_mesh_apply_plan_speeds(fim_mesh *m, const char *plan_path, int64_t *n_set) {
+  return guarded([&] {
+    if (!m || !plan_path) throw Error(FIM_EINVAL, "null argument");
+    Options o;
+    const Plan plan = load_plan(plan_path, o);
+    Mesh &mm = m->m;
+    int64_t cnt = 0;
+    for (int64_t e = 0; e < mm.ne; ++e) {
+      const int32_t tag = mm.etags.empty() ? 0 : mm.etags[e];
+      for (const Region &r : plan.regions)
+        if (std::find(r.tags.begin(), r.tags.end(), tag) != r.tags.end()) {
+          if (!(r.vf > 0) || !(r.vs > 0)) throw Error(FIM_EINVAL, "plan speeds must be positive");
+          mm.vel[2 * e] = r.vf;
+          mm.vel[2 * e + 1] = r.vs;
+          ++cnt;
+          break;
+        }
+    }
+    if (n_set) *n_set = cnt;
+  });
+}
 
 int fim_config_load_options(const char *cfg_path, fim_options_t *out) {
   return guarded([&] {

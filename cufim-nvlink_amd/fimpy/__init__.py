@@ -105,6 +105,8 @@ SIGNATURES = {
     "fim_pack_info": (C.c_int, [_vp, C.POINTER(PackInfo)]),
     "fim_pack_partition": (C.c_int, [_vp, _pi64, _pi64]),
     "fim_pack_destroy": (None, [_vp]),
+    "fim_pack_save_tags": (C.c_int, [_vp, C.c_char_p]),
+    "fim_mesh_apply_plan_speeds": (C.c_int, [_vp, C.c_char_p, _pi64]),
     "fim_create": (C.c_int, [_vp, C.POINTER(Opts), C.POINTER(_vp)]),
     "fim_reset": (C.c_int, [_vp]),
     "fim_seed": (C.c_int, [_vp, C.c_int32, _pi64, C.c_int64, C.c_double]),
@@ -239,6 +241,11 @@ class Mesh:
     def front_width(self):
         return lib().fim_mesh_front_width(self.h)
 
+    def apply_plan_speeds(self, plan_path):
+        n = C.c_int64()
+        _check(lib().fim_mesh_apply_plan_speeds(self.h, plan_path.encode(), C.byref(n)))
+        return n.value
+
     def save_vtk(self, path, phi=None, state=None):
         p = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
         s = None if state is None else np.ascontiguousarray(state, dtype=np.int32)
@@ -264,6 +271,9 @@ class Pack:
         c2g = np.zeros(self.info["ncopy"], dtype=np.int64)
         _check(lib().fim_pack_partition(self.h, _p(e2s, _pi64), _p(c2g, _pi64)))
         return e2s, c2g
+
+    def save_tags(self, path):
+        _check(lib().fim_pack_save_tags(self.h, path.encode()))
 
     def __del__(self):
         if getattr(self, "h", None) and _LIB is not None:

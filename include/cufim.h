@@ -105,6 +105,13 @@ double fim_mesh_front_width(const fim_mesh *m);
  * phi / states (values >= 1e50 written as -1).  phi/state may be NULL. */
 int fim_mesh_save_vtk(const fim_mesh *m, const char *path, const double *phi,
                       const int32_t *state);
+/* Conduction velocities from a plan: every element whose tag is listed by a
+ * region (plan "config" -> "functions"[f].CV.CVref) gets v_fibre = vf and
+ * v_cross = vs (the metric is transversely isotropic; vn is not used).  The
+ * reference parses these (src/io/io_json.cpp:82-128) but solves with a fixed
+ * 0.6 m/s (src/core/types.cpp:231); this entry point is opt-in and the
+ * drop-in path does not call it.  n_set = number of elements changed. */
+int fim_mesh_apply_plan_speeds(fim_mesh *m, const char *plan_path, int64_t *n_set);
 void fim_mesh_destroy(fim_mesh *m);
 
 /* Synthetic meshes (own generators, seeded, deterministic).
@@ -161,6 +168,11 @@ int fim_pack_info(const fim_pack *p, fim_pack_info_t *info);
 /* element -> submesh id (ne) and local->global vertex ids per copy (ncopy) */
 int fim_pack_partition(const fim_pack *p, int64_t *e2s, int64_t *c2g);
 void fim_pack_destroy(fim_pack *p);
+/* writes the element -> submesh map as a tags file (one integer per line,
+ * the format fim_mesh_read_tags and the reference's --part read), so a
+ * partition made here can be fed to both (no counterpart in the reference,
+ * which needs an external METIS run, src/main.cpp:182) */
+int fim_pack_save_tags(const fim_pack *p, const char *path);
 
 /* ========================================================================
  * Device solver context (replaces CudaSolverFIM's device side)

@@ -303,3 +303,44 @@ def test_shell_generator_is_seeded_conforming_and_scarred():
     assert np.abs(np.einsum("ij,ij->i", n, fa)).max() < 0.1  # normal taken at the hex centre
     sites = W.inner_surface_sites(a, 3, 7)
     assert len(set(sites.tolist())) == 3
+
+
+def test_partition_tags_roundtrip(tmp_path):
+    """an own partition written as a tags file and read back as a tag
+    partition gives the same submeshes (the reference's --part input)"""
+    m = F.Mesh.gen_box(seed=3, **cases.scaled(cases.SLAB, 8))
+    a = F.Pack(m, F.PART_RCB, max_nn=200, seed=4)
+    path = str(tmp_path / "rcb.tags")
+    a.save_tags(path)
+    ea, _ = a.partition()
+    assert np.array_equal(np.loadtxt(path, dtype=np.int64), ea)
+    m.read_tags(path)
+    b = F.Pack(m, F.PART_TAGS)
+    eb, _ = b.partition()
+    # tag partitions number submeshes in first-seen tag order (reference
+    # numbering); the grouping is the same
+    assert len(np.unique(eb)) == len(np.unique(ea))
+    pairs = set(zip(ea.tolist(), eb.tolist()))
+    assert len(pairs) == len(np.unique(ea))
+
+
+def test_plan_speeds_by_region(tmp_path):
+    import json
+    m = F.Mesh.load_vtk(UTV)
+    assert m.apply_plan_speeds(f"{cases.UT}/ut_mesh.plan.json") == m.ne
+    _, vel = m.material()
+    assert np.all(vel == 0.6)
+    plan = json.load(open(f"{cases.UT}/ut_mesh.plan.json"))
+    f2 = json.loads(json.dumps(plan["functions"]["fnorm"]))
+    f2["CV"]["CVref"].update(vf=0.9, vs=0.3)
+    plan["functions"]["fslow"] = f2
+    plan["config"]["norm"]["tags"] = [1, 2]
+    plan["config"]["slow"] = {"func": "fslow", "tags": [3, 4]}
+    p = tmp_path / "p.json"
+    p.write_text(json.dumps(plan))
+    m2 = F.Mesh.load_vtk(UTV)
+    assert m2.apply_plan_speeds(str(p)) == m2.ne
+    _, _, tags = m2.arrays()
+    _, vel2 = m2.material()
+    slow = np.isin(tags, [3, 4])
+    assert np.all(vel2[slow] == [0.9, 0.3]) and np.all(vel2[~slow] == 0.6)
