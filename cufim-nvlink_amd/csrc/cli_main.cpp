@@ -19,7 +19,8 @@ namespace {
 void usage() {
   std::printf(
       "Usage: cufim [--help] [--mesh STR] [--part STR] [--plan STR] [--odir STR]\n"
-      "             [--info] [--test] [--reps INT] [--num_threads INT] [--cfg STR] [--device INT]\n\n"
+      "             [--info] [--test] [--reps INT] [--num_threads INT] [--cfg STR] [--device INT]\n"
+      "             [--timed-events]\n\n"
       "FIM activation-time solver for tetrahedral meshes on one MI355X.\n\n"
       "  -m, --mesh         mesh file (legacy ASCII VTK)\n"
       "  -p, --part         partitioning file (.tags, one integer per element)\n"
@@ -30,7 +31,9 @@ void usage() {
       "  -r, --reps         number of runs/repetitions [default: 1]\n"
       "  -np, --num_threads accepted for compatibility (unused) [default: 8]\n"
       "  --cfg              options file [default: data/default.cfg]\n"
-      "  --device           GPU ordinal [default: 0]\n");
+      "  --device           GPU ordinal [default: 0]\n"
+      "  --timed-events     fire each event in the time window of its start time (multi-beat\n"
+      "                     pacing); by default every event fires at t = 0, as in cufim-nvlink\n");
 }
 
 // directory of this executable's repository tests/golden/ut_mesh, for --test
@@ -63,12 +66,13 @@ int main(int argc, char **argv) {
     if (al != alias.end()) key = al->second;
     if (key != "--mesh" && key != "--part" && key != "--plan" && key != "--odir" &&
         key != "--info" && key != "--test" && key != "--reps" && key != "--num_threads" &&
-        key != "--help" && key != "--cfg" && key != "--device") {
+        key != "--help" && key != "--cfg" && key != "--device" && key != "--timed-events") {
       std::fprintf(stderr, "Unknown argument: %s\n", a.c_str());
       usage();
       return EXIT_FAILURE;
     }
-    if (eq == std::string::npos && key != "--info" && key != "--test" && key != "--help") {
+    if (eq == std::string::npos && key != "--info" && key != "--test" && key != "--help" &&
+        key != "--timed-events") {
       if (i + 1 >= argc) {
         std::fprintf(stderr, "%s: expected 1 argument\n", key.c_str());
         return EXIT_FAILURE;
@@ -97,7 +101,7 @@ int main(int argc, char **argv) {
     } else if (args.count("--mesh") && args.count("--part") && args.count("--plan") &&
                args.count("--odir")) {
       const double s = fim::run(args["--mesh"], args["--part"], args["--plan"], args["--odir"],
-                                reps, cfg, device);
+                                reps, cfg, device, args.count("--timed-events") > 0);
       std::printf("Average execution time for %d runs: %f s\n", reps, s);
     } else {
       usage();

@@ -55,15 +55,20 @@ double SolverFIM::solve(std::vector<std::vector<double>> &, std::vector<std::vec
     ck(fim_reset(ctx_));
     // events fire at cur_time 0; the scan stops at the first event that is
     // inactive or starts before 0 (cuda_fim.cu:636-640)
-    for (const Event &ev : events_) {
-      if (!(ev.tstart >= 0.0 && ev.active)) break;
-      ck(fim_seed(ctx_, 0, ev.vtx.data(), (int64_t)ev.vtx.size(), 0.0));
-    }
+    if (!timed_events_)
+      for (const Event &ev : events_) {
+        if (!(ev.tstart >= 0.0 && ev.active)) break;
+        ck(fim_seed(ctx_, 0, ev.vtx.data(), (int64_t)ev.vtx.size(), 0.0));
+      }
     double cur = 0.0;
     const auto t0 = std::chrono::steady_clock::now();
     for (int64_t step = 0; step < opts_.n_steps; ++step) {
       const double phi_start = cur;
       cur += opts_.dt;
+      if (timed_events_)
+        for (const Event &ev : events_)
+          if (ev.active && ev.tstart >= phi_start && ev.tstart < cur)
+            ck(fim_seed(ctx_, 0, ev.vtx.data(), (int64_t)ev.vtx.size(), ev.tstart));
       ck(fim_step(ctx_, phi_start, cur, opts_.dt, front_width_, nullptr));
     }
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -82,7 +87,7 @@ double SolverFIM::solve(std::vector<std::vector<double>> &, std::vector<std::vec
 // src/main.cpp:84-104
 double run(const std::string &mesh_path, const std::string &part_path,
            const std::string &plan_path, const std::string &out_path, int num_reps,
-           const std::string &cfg_path, int device) {
+           const std::string &cfg_path, int device, bool timed_events) {
   fim_mesh *m = nullptr;
   ck(fim_mesh_load_vtk(mesh_path.c_str(), &m));
   struct Guard {
@@ -94,6 +99,7 @@ double run(const std::string &mesh_path, const std::string &part_path,
   Plan plan = load_plan(plan_path, opts);
   std::vector<std::vector<double>> act, apd;
   SolverFIM solver(m, opts, plan.events, device);
+  solver.set_timed_events(timed_events);
   return solver.solve(act, apd, out_path, num_reps);
 }
 

@@ -78,6 +78,13 @@ class SolverFIM {
     return solve(act_ms, apd_ms, file, num_reps);
   }
 
+  // Timed events (off by default, as in the reference): event i fires in
+  // the time window that holds its start time, with phi = tstart, and every
+  // active event fires -- multi-beat pacing with re-entry through the
+  // tail -> far states.  The reference fires every event at t = 0
+  // (cuda_fim.cu:632-683).
+  void set_timed_events(bool on) { timed_events_ = on; }
+
   const std::vector<double> &phi() const { return phi_; }
   const std::vector<int32_t> &states() const { return states_; }
   double front_width() const { return front_width_; }
@@ -85,6 +92,7 @@ class SolverFIM {
  private:
   fim_mesh *mesh_;
   Options opts_;
+  bool timed_events_ = false;
   std::vector<Event> events_;
   fim_pack *pack_ = nullptr;
   fim_ctx *ctx_ = nullptr;
@@ -97,6 +105,7 @@ class SolverFIM {
 // the reference does), plan; solve; write `out_path`.  Returns mean seconds.
 double run(const std::string &mesh_path, const std::string &part_path,
            const std::string &plan_path, const std::string &out_path, int num_reps,
-           const std::string &cfg_path = "data/default.cfg", int device = 0);
+           const std::string &cfg_path = "data/default.cfg", int device = 0,
+           bool timed_events = false);
 
 }  // namespace fim

@@ -578,10 +578,34 @@ static void compute_step(sub_t *s, int variant, double fw, double cur_time,
   }
 }
 
+static int compat_solve(const orc_mesh *m, const orc_events *ev, double dt, int64_t n_steps,
+                        double apd, double front_width, int variant, double *phi_out,
+                        int32_t *state_out, double *stats, int timed);
+
 int orc_compat_solve(const orc_mesh *m, const orc_events *ev, double dt,
                      int64_t n_steps, double apd, double front_width,
                      int variant, double *phi_out, int32_t *state_out,
                      double *stats) {
+  return compat_solve(m, ev, dt, n_steps, apd, front_width, variant, phi_out, state_out, stats, 0);
+}
+
+/* Timed events: the same windows, but event i fires in the window
+ * [phi_start, cur) that contains its tstart, seeding its vertices with
+ * phi = tstart and state narrow before that window's sync; every active
+ * event fires (no stop at the first miss).  This is the multi-beat
+ * behaviour the reference's update_events_gpu (cuda_fim.cu:632-683) would
+ * give if it compared tstart with the window rather than firing everything
+ * at cur_time 0; it has no golden and is pinned by the host-side checks in
+ * tests/test_oracle.py and the GPU parity test. */
+int orc_compat_solve_timed(const orc_mesh *m, const orc_events *ev, double dt, int64_t n_steps,
+                           double apd, double front_width, int variant, double *phi_out,
+                           int32_t *state_out, double *stats) {
+  return compat_solve(m, ev, dt, n_steps, apd, front_width, variant, phi_out, state_out, stats, 1);
+}
+
+static int compat_solve(const orc_mesh *m, const orc_events *ev, double dt, int64_t n_steps,
+                        double apd, double front_width, int variant, double *phi_out,
+                        int32_t *state_out, double *stats, int timed) {
   const int64_t ne = m->ne, nn = m->nn;
   if (ne <= 0 || nn <= 0 || !m->atags) return -1;
   int64_t *e2n = (int64_t *)malloc(sizeof(int64_t) * 4 * ne);
@@ -672,7 +696,7 @@ int orc_compat_solve(const orc_mesh *m, const orc_events *ev, double dt,
       subs[a].state[l] = ORC_FAR;
     }
   /* update_events_gpu at cur_time 0 (cuda_fim.cu:632-683) */
-  if (ev)
+  if (ev && !timed)
     for (int32_t i = 0; i < ev->nev; ++i) {
       if (!(ev->tstart[i] >= 0.0 && ev->active[i])) break;
       for (int64_t j = ev->dsp[i]; j < ev->dsp[i + 1]; ++j) {
@@ -689,6 +713,18 @@ int orc_compat_solve(const orc_mesh *m, const orc_events *ev, double dt,
   for (int64_t step = 0; step < n_steps; ++step) {
     const double phi_start = cur;
     cur += dt;
+    if (ev && timed)
+      for (int32_t i = 0; i < ev->nev; ++i) {
+        if (!ev->active[i] || !(ev->tstart[i] >= phi_start && ev->tstart[i] < cur)) continue;
+        for (int64_t j = ev->dsp[i]; j < ev->dsp[i + 1]; ++j) {
+          const int64_t v = ev->vtx[j];
+          if (v < 0 || v >= nn) continue;
+          for (int64_t p = ci_cnt[v]; p < ci_cnt[v + 1]; ++p) {
+            subs[ci_a[p]].phi[ci_l[p]] = ev->tstart[i];
+            subs[ci_a[p]].state[ci_l[p]] = ORC_NARROW;
+          }
+        }
+      }
     /* sync_step2 (cuda_fim.cu:1752-1787), per global vertex */
     for (int64_t v = 0; v < nn; ++v) {
       double mn = DMAX;

@@ -141,6 +141,13 @@ int orc_compat_solve(const orc_mesh *m, const orc_events *ev, double dt,
                      int variant, double *phi_out, int32_t *state_out,
                      double *stats);
 
+/* Same windows with timed events: event i fires in the window that holds
+ * its tstart (phi = tstart), every active event fires. */
+int orc_compat_solve_timed(const orc_mesh *m, const orc_events *ev, double dt,
+                           int64_t n_steps, double apd, double front_width,
+                           int variant, double *phi_out, int32_t *state_out,
+                           double *stats);
+
 /* FP64 label-correcting fixed point of the local update (plain FIM
  * semantics: every finite neighbour is usable).  Sources get value t0[i].
  * Returns 0 on success.  stats (may be NULL): [0] heap pops, [1] local solves. */

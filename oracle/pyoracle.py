@@ -72,6 +72,10 @@ def lib():
         L.orc_compat_solve.argtypes = [C.POINTER(_Mesh), C.POINTER(_Events), C.c_double,
                                        C.c_int64, C.c_double, C.c_double, C.c_int,
                                        _pd, _pi32, _pd]
+        L.orc_compat_solve_timed.restype = C.c_int
+        L.orc_compat_solve_timed.argtypes = [C.POINTER(_Mesh), C.POINTER(_Events), C.c_double,
+                                       C.c_int64, C.c_double, C.c_double, C.c_int,
+                                       _pd, _pi32, _pd]
         L.orc_fim_solve.restype = C.c_int
         L.orc_fim_solve.argtypes = [C.POINTER(_Mesh), C.c_int64, _pi64, _pd, C.c_int,
                                     _pd, _pd]
@@ -195,13 +199,14 @@ def _events(stims):
 
 
 def compat_solve(mesh: Mesh, stims, dt, n_steps, apd=3e5, front_width=-1.0,
-                 variant=LOCAL_REF):
+                 variant=LOCAL_REF, timed=False):
     ev, keep = _events(stims)
     nn = mesh.xyz.shape[0]
     phi = np.zeros(nn)
     st = np.zeros(nn, dtype=np.int32)
     stats = np.zeros(2)
-    rc = lib().orc_compat_solve(C.byref(mesh.c), C.byref(ev), dt, n_steps, apd, front_width,
+    fn = lib().orc_compat_solve_timed if timed else lib().orc_compat_solve
+    rc = fn(C.byref(mesh.c), C.byref(ev), dt, n_steps, apd, front_width,
                                 variant, _p(phi, _pd), _p(st, _pi32), _p(stats, _pd))
     if rc != 0:
         raise RuntimeError(f"orc_compat_solve failed: {rc}")

@@ -275,3 +275,32 @@ def test_scarred_shell_vs_fp64_oracle():
     phi, st, it, res, stats = _converge(m, p, sites, window=2000.0)
     ref, _ = O.fim_solve(om, sites, variant=O.LOCAL_EXACT)
     _check_fp32(phi, ref)
+
+
+def test_compat_timed_events_two_beats_bit_identical():
+    """multi-beat pacing with events fired in the window holding their start
+    time (re-entry through tail -> far): GPU compat mode equals the oracle's
+    timed restatement bit for bit"""
+    cfg = cases.scaled(cases.CUBE, 2)
+    m, om = cases.box(cfg, seed=3)
+    p = F.Pack(m, F.PART_RCB, max_nn=128, seed=3)
+    e2s, _ = p.partition()
+    om = O.Mesh(om.xyz, om.e2n, fib=om.fib, vel=om.vel, atags=e2s.astype(np.int32))
+    dt, apd, steps = 10e3, 50e3, 18
+    stims = [(1, 0.0, [0]), (1, 120e3, [0]), (1, 125e3, [1330])]
+    s = F.Solver(p, mode=F.MODE_COMPAT, local=F.LOCAL_REF, apd=apd)
+    fw = m.front_width()
+    s.reset()
+    cur = 0.0
+    for _ in range(steps):
+        ps = cur
+        cur += dt
+        for active, tstart, vtx in stims:
+            if active and ps <= tstart < cur:
+                s.seed(vtx, tstart)
+        s.step(ps, cur, dt, fw)
+    phi, st = s.download()
+    s.close()
+    ophi, ost, _ = O.compat_solve(om, stims, dt, steps, apd=apd, front_width=fw, timed=True)
+    assert np.array_equal(st, ost)
+    assert np.array_equal(phi, ophi)

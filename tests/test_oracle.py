@@ -374,3 +374,60 @@ def test_fixed_point_is_order_independent():
             break
         cur = new
     assert np.max(np.abs(cur - phi)) <= 1e-9 * phi.max()
+
+
+def _ut_compat(stims, dt, n_steps, timed):
+    m = O.Mesh(*_ut_mesh_arrays())
+    return O.compat_solve(m, stims, dt, n_steps, timed=timed)
+
+
+def _ut_mesh_arrays():
+    import vtkio
+    g = vtkio.read_vtk(f"{cases.UT}/ut_mesh.vtk")
+    tags = np.loadtxt(f"{cases.UT}/ut_mesh.tags", dtype=np.int32)
+    return g["xyz"], g["cells"], None, None, tags
+
+
+def test_timed_events_reduce_to_the_reference_at_time_zero():
+    """one event at t = 0: firing in the first window equals the reference's
+    seeding before the loop (the golden-pinned path)"""
+    stims = cases.ut_plan_stimuli()
+    a = _ut_compat(stims, 150e3, 3, timed=False)
+    b = _ut_compat(stims, 150e3, 3, timed=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_timed_event_shifted_by_whole_windows_shifts_the_map():
+    """an event at 2 windows fires two windows later and gives the same map
+    shifted by 2 dt (window boundaries line up with the event)"""
+    (act, t0, vtx), = cases.ut_plan_stimuli()
+    dt = 150e3
+    a_phi, a_st, _ = _ut_compat([(act, 0.0, vtx)], dt, 3, timed=True)
+    b_phi, b_st, _ = _ut_compat([(act, 2 * dt, vtx)], dt, 5, timed=True)
+    fin = a_phi < 1e50
+    assert np.array_equal(fin, b_phi < 1e50) and np.array_equal(a_st, b_st)
+    # equal up to the reference update's FP32 intermediates (a few float ulps
+    # at these times); a wrong window or seed time would be off by ~dt
+    assert np.max(np.abs(b_phi[fin] - (a_phi[fin] + 2 * dt))) < 1e-6 * (b_phi[fin].max())
+
+
+def test_second_beat_after_recovery_repeats_the_first():
+    """on a small cube (front crosses in ~30 ms) with a 50 ms action
+    potential: a second beat 120 ms after the first finds the tissue far
+    again (tail -> far at phi + APD / 0.91) and the final map is the first
+    beat's map shifted by the cycle length; a single beat ends elsewhere"""
+    import fimpy as F
+    cfg = cases.scaled(cases.CUBE, 2)
+    m, om = cases.box(cfg, seed=3)
+    e2s, _ = F.Pack(m, F.PART_RCB, max_nn=128, seed=3).partition()
+    om = O.Mesh(om.xyz, om.e2n, fib=om.fib, vel=om.vel, atags=e2s.astype(np.int32))
+    dt, bcl, apd = 10e3, 120e3, 50e3
+    ev = (1, 0.0, [0])
+    one_phi, one_st, _ = O.compat_solve(om, [ev], dt, 6, apd=apd, timed=True)
+    two_phi, two_st, _ = O.compat_solve(om, [ev, (1, bcl, [0])], dt, 18, apd=apd, timed=True)
+    fin = one_phi < 1e50  # the earliest vertices have recovered (far, phi unset) by 60 ms
+    assert 0.5 * om.xyz.shape[0] < fin.sum() < om.xyz.shape[0]
+    assert np.array_equal(fin, two_phi < 1e50) and np.array_equal(one_st, two_st)
+    assert np.max(np.abs(two_phi[fin] - (one_phi[fin] + bcl))) < 1e-6 * two_phi[fin].max()
+    lone_phi, lone_st, _ = O.compat_solve(om, [ev], dt, 18, apd=apd, timed=True)
+    assert not np.array_equal(lone_st, two_st)
