@@ -12,14 +12,14 @@ fixed point of the CPU oracle (max |dT| <= 0.1 ms is required for a valid
 number), and that oracle run, timed on one host core, is the CPU baseline.
 Plain single-solve GPU FIM (B = 1) is timed beside it.
 
-The surrogate warm start (coordinate MLP on BF16 MFMA) is measured in the
-same run on the same slab with quadrant speed factors: 4,096 training maps
+With --nn the surrogate warm start (coordinate MLP on BF16 MFMA) is measured
+in the same run on the same slab with quadrant speed factors: 4,096 training maps
 from the batched solver, Adam training, the upward shift delta calibrated on
 64 validation solves, then 512 held-out solves warm-started (prediction +
 delta, all vertices active, support check as judge, cold re-solve as
 fallback) against the same 512 solved cold.  Its numbers are reported in
 `nn_warm_start`; the headline value is the batched solver alone, which is
-the faster of the two on this workload.
+the faster of the two on this workload (the warm start is 3x slower).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
@@ -71,7 +71,7 @@ def traffic_from_profiles(args):
     c = re.search(r"traffic_config:\s*(.*)", text)
     mine = (f"batch={args.batch} max_nn={args.max_nn} block={args.block} unit={args.unit} "
             f"local_iters={args.local_iters} "
-            f"window={float(args.window)} staged={args.staged} nn={not args.no_nn}")
+            f"window={float(args.window)} staged={args.staged} nn={args.nn}")
     src = re.search(r"traffic_source:\s*([0-9a-f]+)", text)
     if not m or not c or c.group(1).strip() != mine or not src or src.group(1) != kernel_source_digest():
         return None
@@ -107,7 +107,12 @@ def nn_leg(args, account, plain_s):
     t_data = WS.build_dataset(nn, pack, train, unit=args.unit, **kw)
     log(f"[bench] surrogate dataset: {args.nn_maps} maps in {t_data:.2f}s")
     t0 = time.perf_counter()
-    loss = nn.train(args.nn_steps, batch=args.nn_batch, seed=args.seed)
+    done, loss = 0, float("nan")
+    while done < args.nn_steps:
+        k = min(500, args.nn_steps - done)
+        loss = nn.train(k, batch=args.nn_batch, seed=args.seed * 7919 + done)
+        done += k
+        log(f"[bench] surrogate training {done}/{args.nn_steps} steps, loss {loss:.3e}")
     t_train = time.perf_counter() - t0
     log(f"[bench] surrogate trained: {args.nn_steps} steps of {args.nn_batch} in {t_train:.2f}s, loss {loss:.3e}")
     errs = WS.surrogate_errors(nn, pack, val, **kw)
@@ -205,7 +210,8 @@ def main():
     ap.add_argument("--plain-max-nn", type=int, default=256)
     ap.add_argument("--plain-window", type=float, default=2000.0)
     ap.add_argument("--plain-local-iters", type=int, default=4)
-    ap.add_argument("--no-nn", action="store_true", help="skip the surrogate warm-start leg")
+    ap.add_argument("--nn", action="store_true",
+                    help="also measure the surrogate warm start (adds about a minute)")
     ap.add_argument("--nn-maps", type=int, default=4096)
     ap.add_argument("--nn-steps", type=int, default=4000)
     ap.add_argument("--nn-batch", type=int, default=65536)
@@ -256,6 +262,7 @@ def main():
         account(st)
         for k in tot:
             tot[k] += st[k]
+        log(f"[bench] step {i + 1}/{args.steps}")
     elapsed = time.perf_counter() - t0
     log(f"[bench] timed {args.steps} steps of {args.batch} maps in {elapsed:.3f}s")
     phi0, _ = s.download(inst=0)
@@ -298,7 +305,7 @@ def main():
     log(f"[bench] plain single solve {1e3 * plain_s:.2f} ms")
 
     # ---- surrogate warm start ---------------------------------------------
-    nnrep = None if args.no_nn else nn_leg(args, account, plain_s)
+    nnrep = nn_leg(args, account, plain_s) if args.nn else None
 
     # ---- accuracy and CPU baseline (FP64 oracle, one host core) ---------
     max_dt = None
@@ -343,7 +350,7 @@ def main():
             "workload": "anisotropic slab 160x160x13 hexes x 6 = 1,996,800 tets / 362,894 vertices, "
                         "fibres -60..+60 deg, v_f 0.6 / v_c 0.24 m/s, 5 pacing sites per map; "
                         f"batch of {args.batch} maps per step solved to eps 1e-6 "
-                        "(batched FIM; the surrogate warm start is measured in nn_warm_start)",
+                        "(batched FIM; the surrogate warm start, slower here, is measured with --nn)",
             "maps_per_step": args.batch,
             "partition": f"recursive bisection, <= {args.max_nn} vertex copies per submesh, "
                          f"{info['nsub']} submeshes",

@@ -304,3 +304,40 @@ def test_compat_timed_events_two_beats_bit_identical():
     ophi, ost, _ = O.compat_solve(om, stims, dt, steps, apd=apd, front_width=fw, timed=True)
     assert np.array_equal(st, ost)
     assert np.array_equal(phi, ophi)
+
+
+def test_converge_edge_cases():
+    """a seed time shifts the map; an unreached component stays unreached;
+    an iteration cap reports FIM_ENOCONV and a further run finishes the
+    same map; a submesh too large for LDS is refused at creation"""
+    from fimpy import workloads as W
+    cfg = cases.scaled(cases.SLAB, 8)
+    a = F.Mesh.gen_box(seed=1, **cfg)
+    b = F.Mesh.gen_box(seed=2, **cases.scaled(cases.CUBE, 4))
+    u = F.Mesh.concat([a, b])
+    p = F.Pack(u, F.PART_RCB, max_nn=256, seed=1)
+    src = cases.pacing_sites(cfg, 2, seed=3)
+    s = F.Solver(p, n_instances=3, window=1000.0)
+    s.seed(src, 0.0, inst=0)
+    s.seed(src, 5000.0, inst=1)
+    s.run()
+    p0, _ = s.download(inst=0)
+    p1, _ = s.download(inst=1)
+    p2, st2 = s.download(inst=2)  # never seeded
+    fin = p0 < 1e50
+    assert np.array_equal(np.flatnonzero(fin), np.arange(a.nn))  # only the first component
+    assert np.allclose(p1[fin], p0[fin] + 5000.0, rtol=0, atol=1e-4 * p0[fin].max())
+    assert np.all(p2 >= 1e50) and np.all(st2 == F.FAR)
+    # iteration cap, then completion
+    s.reset()
+    s.seed(src, 0.0, inst=0)
+    with pytest.raises(F.FimError, match="FIM_ENOCONV"):
+        s.run(max_iters=3)
+    s.run()
+    again, _ = s.download(inst=0)
+    assert np.array_equal(again, p0)
+    s.close()
+    # LDS limit: one submesh holding the whole 2M-tet slab's worth of copies
+    big = F.Pack(F.Mesh.gen_box(seed=1, **cases.scaled(cases.SLAB, 2)), F.PART_RCB, max_nn=60000, seed=1)
+    with pytest.raises(F.FimError, match="FIM_ELIMIT"):
+        F.Solver(big)
