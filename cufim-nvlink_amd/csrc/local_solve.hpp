@@ -107,6 +107,40 @@ FIM_HD inline T cand_core(T e12, T e13, T e23, T e14, T e24, T e34, T p1, T p2, 
   return best;
 }
 
+// tri_exact with the reciprocal of G = e12 given
+template <class T>
+FIM_HD inline T tri_exact_r(T e12, T rG, T e14, T e24, T p1, T p2, T inf) {
+  const T G = e12, g = T(0.5) * (e12 + e14 - e24), d = p2 - p1;
+  const T den = G - d * d;
+  const T r = e14 - g * g * rG;
+  const T s = csqrt(std::fmax(cdiv(r * G, den), T(0)));
+  const T l = (g - s * d) * rG;
+  const bool ok = (den > T(0)) && (r > T(0)) && (l > T(0)) && (l < T(1));
+  return ok ? p1 + l * d + s : inf;
+}
+
+// candidate for node position TP with per-element square roots sq[k] and
+// reciprocals rc[k] of the six squared lengths
+template <int TP, class T>
+FIM_HD inline T cand_exact_sh(const T g[6], const T sq[6], const T rc[6], const T ph[4], T inf) {
+  constexpr int q1 = (TP + 1) & 3, q2 = (TP + 2) & 3, q3 = (TP + 3) & 3;
+  constexpr int i12 = edge_idx(q1, q2), i13 = edge_idx(q1, q3), i23 = edge_idx(q2, q3);
+  constexpr int i14 = edge_idx(q1, TP), i24 = edge_idx(q2, TP), i34 = edge_idx(q3, TP);
+  const T p1 = ph[q1], p2 = ph[q2], p3 = ph[q3];
+  const bool u1 = p1 < inf, u2 = p2 < inf, u3 = p3 < inf;
+  T best = std::fmin(std::fmin(u1 ? p1 + sq[i14] : inf, u2 ? p2 + sq[i24] : inf),
+                     u3 ? p3 + sq[i34] : inf);
+  const T t12 = tri_exact_r<T>(g[i12], rc[i12], g[i14], g[i24], p1, p2, inf);
+  const T t13 = tri_exact_r<T>(g[i13], rc[i13], g[i14], g[i34], p1, p3, inf);
+  const T t23 = tri_exact_r<T>(g[i23], rc[i23], g[i24], g[i34], p2, p3, inf);
+  const T t123 = tet_exact<T>(g[i12], g[i13], g[i23], g[i14], g[i24], g[i34], p1, p2, p3, inf);
+  best = std::fmin(best, (u1 && u2) ? t12 : inf);
+  best = std::fmin(best, (u1 && u3) ? t13 : inf);
+  best = std::fmin(best, (u2 && u3) ? t23 : inf);
+  best = std::fmin(best, (u1 && u2 && u3) ? t123 : inf);
+  return best;
+}
+
 // candidate for node position TP (compile time)
 template <int TP, class T>
 FIM_HD inline T cand_exact(const T g[6], const T ph[4], T inf) {

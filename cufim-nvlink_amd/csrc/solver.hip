@@ -747,6 +747,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         const float g[6] = {__uint_as_float(r0.z) * sc, __uint_as_float(r0.w) * sc,
                             __uint_as_float(r1.x) * sc, __uint_as_float(r1.y) * sc,
                             __uint_as_float(r1.z) * sc, __uint_as_float(r1.w) * sc};
+        // square roots (edge candidates) and reciprocals (triangle bases) of
+        // the six squared lengths, shared by the element's four targets
+        float sq[6], rc[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          sq[k] = csqrt(g[k]);
+          rc[k] = __builtin_amdgcn_rcpf(g[k]);
+        }
         const float hs = csqrt(sc);
         const float h[4] = {__uint_as_float(r2.x) * hs, __uint_as_float(r2.y) * hs,
                             __uint_as_float(r2.z) * hs, __uint_as_float(r2.w) * hs};
@@ -764,7 +772,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         // other node + metric height) is still below their time
 #define FIM_TARGET(T, A, B, C)                                                        \
         if (nf - f[T] > 0 && ph[T] > fminf(ph[A], fminf(ph[B], ph[C])) + h[T]) {         \
-          const float c = cand_exact<T, float>(g, ph, F_INF);                            \
+          const float c = cand_exact_sh<T, float>(g, sq, rc, ph, F_INF);                 \
           ++vupd;                                                                         \
           if (c < ph[T]) {                                                                \
             const uint32_t cbits = __float_as_uint(c);                                    \

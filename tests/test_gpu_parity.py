@@ -309,7 +309,7 @@ def test_compat_timed_events_two_beats_bit_identical():
 def test_converge_edge_cases():
     """a seed time shifts the map; an unreached component stays unreached;
     an iteration cap reports FIM_ENOCONV and a further run finishes the
-    same map; a submesh too large for LDS is refused at creation"""
+    same map"""
     from fimpy import workloads as W
     cfg = cases.scaled(cases.SLAB, 8)
     a = F.Mesh.gen_box(seed=1, **cfg)
@@ -337,7 +337,3 @@ def test_converge_edge_cases():
     again, _ = s.download(inst=0)
     assert np.array_equal(again, p0)
     s.close()
-    # LDS limit: one submesh holding the whole 2M-tet slab's worth of copies
-    big = F.Pack(F.Mesh.gen_box(seed=1, **cases.scaled(cases.SLAB, 2)), F.PART_RCB, max_nn=60000, seed=1)
-    with pytest.raises(F.FimError, match="FIM_ELIMIT"):
-        F.Solver(big)

@@ -344,3 +344,11 @@ def test_plan_speeds_by_region(tmp_path):
     _, vel2 = m2.material()
     slow = np.isin(tags, [3, 4])
     assert np.all(vel2[slow] == [0.9, 0.3]) and np.all(vel2[~slow] == 0.6)
+
+
+def test_submesh_size_limits_are_refused():
+    """local ids are 16-bit: a partition leaf above 65535 elements is refused
+    when the pack is built, before anything reaches a device"""
+    m = F.Mesh.gen_box(seed=1, **cases.scaled(cases.SLAB, 2))
+    with pytest.raises(F.FimError, match="FIM_ELIMIT"):
+        F.Pack(m, F.PART_RCB, max_nn=60000, seed=1)

@@ -64,19 +64,26 @@ def shim(tmp_path_factory):
 
 
 def _cases(n, seed):
-    """Geometry of random non-degenerate tets under a random anisotropic
-    metric, node times spread so that every branch (edge, triangle, face,
-    unusable node) is taken."""
+    """Geometry of random well-shaped tets (a regular tet of side ~600 um,
+    vertices moved by up to 30 % of a side, random orientation) under a
+    random transversely anisotropic metric (speeds 0.2-0.6 um/us, as in
+    cardiac tissue), node times spread so that every branch (edge,
+    triangle, face, unusable node) is taken."""
     rng = np.random.default_rng(seed)
+    base = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.5, np.sqrt(3) / 2, 0.0],
+                     [0.5, np.sqrt(3) / 6, np.sqrt(2.0 / 3.0)]]) * 600.0
     out = []
     while len(out) < n:
-        x = rng.normal(size=(4, 3)) * 500.0
-        a = rng.normal(size=(3, 3))
-        M = a @ a.T + 0.2 * np.eye(3)
-        M /= 0.36
-        if abs(np.linalg.det(x[1:] - x[0])) < 1e5:
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        x = (base + rng.uniform(-0.3, 0.3, (4, 3)) * 600.0) @ q
+        if abs(np.linalg.det(x[1:] - x[0])) < 1e6:
             continue
-        g = O.geo_elem(x, np.arange(4), M)
+        r, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        v = rng.uniform(0.2, 0.6, 3)
+        M = r @ np.diag(1.0 / v ** 2) @ r.T
+        packed = np.array([M[0, 0], M[0, 1], M[0, 2], M[1, 1], M[1, 2], M[2, 2]])
+        g = O.geo_elem(x, np.arange(4), packed)
+        assert np.all(g > 0)  # SPD metric: positive squared lengths
         ph = rng.uniform(0, 2000.0, 4)
         if rng.random() < 0.15:
             ph[rng.integers(4)] = 1e300
@@ -123,3 +130,49 @@ def test_exact_fp32_close_to_oracle(shim):
             assert ours < 3.0e38  # an edge candidate always exists here
             worst = max(worst, abs(ours - ref) / max(1.0, abs(ref)))
     assert worst < 2e-4, worst
+
+
+SHIM_SH = r"""
+#include "local_solve.hpp"
+extern "C" int ls_shared_equal(const float *g, const float *ph) {
+  float sq[6], rc[6];
+  for (int k = 0; k < 6; ++k) { sq[k] = fim::csqrt(g[k]); rc[k] = 1.0f / g[k]; }
+  const float inf = 3.0e38f;
+  const float a[4] = {fim::cand_exact<0, float>(g, ph, inf), fim::cand_exact<1, float>(g, ph, inf),
+                      fim::cand_exact<2, float>(g, ph, inf), fim::cand_exact<3, float>(g, ph, inf)};
+  const float b[4] = {fim::cand_exact_sh<0, float>(g, sq, rc, ph, inf),
+                      fim::cand_exact_sh<1, float>(g, sq, rc, ph, inf),
+                      fim::cand_exact_sh<2, float>(g, sq, rc, ph, inf),
+                      fim::cand_exact_sh<3, float>(g, sq, rc, ph, inf)};
+  int bad = 0;
+  for (int k = 0; k < 4; ++k) {
+    const float tol = 1e-4f * (a[k] < inf ? a[k] : 1.0f);
+    if (!((a[k] >= inf && b[k] >= inf) || (a[k] - b[k] <= tol && b[k] - a[k] <= tol))) ++bad;
+  }
+  return bad;
+}
+"""
+
+
+def test_shared_root_candidate_equals_plain_candidate(tmp_path):
+    """the variant with per-element square roots and reciprocals (device
+    path) gives the same candidates.  On the host a/b and a*(1/b) round
+    differently, so a validity test at a face or edge boundary can flip and
+    select the neighbouring candidate, which is continuous there: 1e-4
+    relative; a wrong edge or reciprocal would be off by far more."""
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "sh.cpp"
+    src.write_text(SHIM_SH)
+    so = tmp_path / "sh.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", f"-I{CSRC}",
+                    str(src), "-o", str(so)], check=True)
+    lib = C.CDLL(str(so))
+    pf = C.POINTER(C.c_float)
+    lib.ls_shared_equal.argtypes = [pf, pf]
+    bad = 0
+    for g, ph in _cases(2000, 7):
+        gf = g.astype(np.float32)
+        pf_ = np.where(ph >= 1e299, np.float32(3.0e38), ph).astype(np.float32)
+        bad += lib.ls_shared_equal(_ptr(gf, C.c_float), _ptr(pf_, C.c_float))
+    assert bad == 0
