@@ -3,20 +3,22 @@
 //
 // Two modes share one pack (host.hpp):
 //
-//  converge (FP32, exact local solver): per global iteration one workgroup
-//    per active (instance, submesh) work item stages the submesh -- element
-//    nodes (4 x u16), metric geometry (6 x f32), vertex times (f32) -- in
-//    LDS and iterates Jacobi sweeps to local convergence.  The active
-//    element set is rebuilt every sweep from a vertex flag bitmask with
-//    wave64 ballots and a per-wave prefix (popcount); candidates are merged
-//    with LDS atomicMin on the float bit pattern (times are >= 0, so the
-//    IEEE order equals the unsigned order).  A second kernel pushes changed
-//    interface copies to the other copies of the vertex with global
-//    atomicMin, marks them dirty and appends their submesh to the next work
-//    list.  Both kernels read only their own work list, so results are
-//    independent of scheduling (atomic min and "set" flags commute).
-//    Iterations are captured in a hipGraph; the host reads one counter per
-//    graph launch.
+//  converge (FP32, exact local solver): per global iteration a persistent
+//    kernel (k_conv_band) works through the (instance, submesh) pairs of the
+//    iteration's work list, dealt out per XCD in units of a few instances of
+//    one submesh.  A work item stages the submesh's vertex times in LDS and
+//    runs Jacobi sweeps (capped per activation): sources mark their incident
+//    elements through a padded vertex -> element adjacency, the marked
+//    elements evaluate exact candidates from 48-byte element records read
+//    from L2, and candidates merge with LDS atomicMin on the float bit
+//    pattern (times are >= 0, so the IEEE order equals the unsigned order).
+//    A second kernel (k_conv_sync) pushes changed interface copies to the
+//    other copies of the vertex with global atomicMin, marks them dirty and
+//    queues their submesh on the next list; a third (k_conv_plan) cuts the
+//    next list into work units.  The kernels read only their own list, so
+//    results are independent of scheduling (atomic min and "set" flags
+//    commute).  Iterations are captured in a hipGraph; the host reads one
+//    counter per graph launch.
 //
 //  compat (FP64, reference local update): one workgroup per submesh runs the
 //    reference's per-window phases a-g (src/eikonal/cuda_fim.cu:1992-2334)
