@@ -191,7 +191,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--max-nn", type=int, default=1024)
+    ap.add_argument("--max-nn", type=int, default=1280)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--local-iters", type=int, default=4,
                     help="sweeps per activation before requeueing (0 = to local convergence)")
@@ -354,6 +354,7 @@ def main():
             "maps_per_step": args.batch,
             "partition": f"recursive bisection, <= {args.max_nn} vertex copies per submesh, "
                          f"{info['nsub']} submeshes",
+            "max_nn": args.max_nn,
             "workgroup": args.block,
             "unit": args.unit,
             "local_iters": args.local_iters,
