@@ -12,10 +12,12 @@
 //    elements evaluate exact candidates from 48-byte element records read
 //    from L2, and candidates merge with LDS atomicMin on the float bit
 //    pattern (times are >= 0, so the IEEE order equals the unsigned order).
-//    A second kernel (k_conv_sync) pushes changed interface copies to the
-//    other copies of the vertex with global atomicMin, marks them dirty and
-//    queues their submesh on the next list; a third (k_conv_plan) cuts the
-//    next list into work units.  The kernels read only their own list, so
+//    A work item writes its lowered interface copies as (entry, time)
+//    records into its own slot range (no global atomics); a second kernel
+//    (k_conv_sync), one wave per work unit, pushes them to the other copies
+//    of the vertex with global atomicMin, marks those dirty and queues their
+//    submesh on the next list; a third (k_conv_plan) cuts the next list into
+//    work units.  The kernels read only their own list, so
 //    results are independent of scheduling (atomic min and "set" flags
 //    commute).  Iterations are captured in a hipGraph; the host reads one
 //    counter per graph launch.
@@ -232,9 +234,14 @@ struct ConvArgs {
   const float *rscale;   // n_inst x 4 geometry multipliers (1/s^2)
   float *phi;            // n_inst x ncopy
   uint8_t *dirty;        // n_inst x ncopy
-  uint8_t *chg;          // n_inst x ncopy
   uint32_t *act;         // n_inst x nsub
   float *wake;           // n_inst x nsub: lower bound of the dirty vertex times
+  const uint16_t *if_l;  // per interface entry: local copy id
+  const uint2 *ifm;      // per submesh, wps words: interface-copy bits of 32
+                         // local copies and the interface entries before them
+  uint2 *prec;           // push records (entry, time bits): instance i,
+                         // submesh s use [i * n_if + ioff(s), + ni(s))
+  uint32_t *pcnt;        // n_inst x nsub: records of the last activation
   WorkList cur;          // work list of this iteration
   WorkList nxt;          // work list of the next iteration (requeues)
   unsigned long long *stats;  // activations, sweeps, element evaluations,
@@ -250,9 +257,41 @@ struct ConvArgs {
   float window;          // per-iteration time window, 0 = off
   int32_t max_local;
   uint32_t unit, maxch;  // instances per work unit, units per submesh bucket
+  uint32_t n_if, wps;    // interface entries; ifm words per submesh
   // LDS carve (bytes)
-  uint32_t o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc, o_wrt;
+  uint32_t o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc, o_wrt, o_ifc;
 };
+
+// Writes the interface copies lowered by this activation (bits in tch) as
+// push records into the item's slot range and their count into pcnt; the
+// entries are found through the interface list (if_l).  Every thread of the
+// workgroup calls it; cnt is one LDS word.
+template <int BS>
+__device__ inline void push_lowered_scan(const ConvArgs &a, const SubInfo &si, uint32_t inst,
+                                         size_t entry, const uint32_t *tch, const float *phi_l,
+                                         uint32_t *cnt) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  if (tid == 0) *cnt = 0u;
+  __syncthreads();
+  uint2 *rec = a.prec + (size_t)inst * a.n_if + si.ioff;
+  for (uint32_t j0 = 0; j0 < si.ni; j0 += BS) {
+    const uint32_t j = j0 + tid;
+    uint32_t l = 0;
+    bool p = false;
+    if (j < si.ni) {
+      l = a.if_l[si.ioff + j];
+      p = flag_at(tch, l);
+    }
+    const uint64_t m = __ballot(p);
+    if (m == 0) continue;  // uniform across the wave
+    uint32_t wb = 0;
+    if (lane == 0) wb = atomicAdd(cnt, (uint32_t)__popcll(m));
+    wb = __shfl(wb, 0);
+    if (p) rec[wb + __popcll(m & lanes_below())] = make_uint2(si.ioff + j, __float_as_uint(phi_l[l]));
+  }
+  __syncthreads();
+  if (tid == 0) a.pcnt[entry] = *cnt;
+}
 
 // One workgroup per work item (instance, submesh).  STAGED copies the
 // submesh's elements (nodes + geometry) into LDS and finds the active
@@ -295,7 +334,10 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
     // nothing in this submesh can act before the window reaches its wake
     // time: keep it queued without staging anything
     if (a.wake[entry] > W) {
-      if (tid == 0) enqueue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
+      if (tid == 0) {
+        enqueue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
+        a.pcnt[entry] = 0u;
+      }
       continue;
     }
     // stage vertex times and dirty flags
@@ -529,21 +571,19 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
       used = misc[3];
     }
     abytes += 48ull * used + 10ull * nn + 12ull * si.ni;
-    // write back decreased copies; remember them for the interface push.
+    // write back decreased copies and push the lowered interface copies.
     // Deferred vertices (and sources left by the sweep cap) stay dirty and
     // the submesh is queued for the next iteration.
     if (tid == 0) misc[1] = __float_as_uint(F_INF);
     __syncthreads();
     for (uint32_t l = tid; l < nn; l += BS) {
-      if (flag_at(tch, l)) {
-        a.phi[cb + l] = phi_l[l];
-        a.chg[cb + l] = 1;
-      }
+      if (flag_at(tch, l)) a.phi[cb + l] = phi_l[l];
       if (flag_at(dfr, l) || (capped && flag_at(flg, l))) {
         a.dirty[cb + l] = 1;
         atomicMin(&misc[1], __float_as_uint(phi_l[l]));
       }
     }
+    push_lowered_scan<BS>(a, si, inst, entry, tch, phi_l, &misc[6]);
     __syncthreads();
     if (tid == 0) {
       const float wk = __uint_as_float(misc[1]);
@@ -648,12 +688,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
   uint32_t *tch = reinterpret_cast<uint32_t *>(lds + a.o_tch);  // decreased this activation
   uint32_t *dfr = reinterpret_cast<uint32_t *>(lds + a.o_dfr);  // flagged, beyond the window
   uint32_t *wrt = reinterpret_cast<uint32_t *>(lds + a.o_wrt);  // lowered in this sweep
+  uint32_t *ifc = reinterpret_cast<uint32_t *>(lds + a.o_ifc);  // interface copies
+  uint32_t *ipre = ifc + a.wps;  // interface entries before each word
   uint32_t *emark = reinterpret_cast<uint32_t *>(lds + a.o_e2n);
   uint32_t *eseen = reinterpret_cast<uint32_t *>(lds + a.o_geo);
   uint16_t *elist = reinterpret_cast<uint16_t *>(lds + a.o_list);
   uint16_t *vlist = reinterpret_cast<uint16_t *>(lds + a.o_vlist);
   // misc: [0] listed elements, [2] listed sources, [3] distinct elements,
-  // [4] work cursor, [5] earliest deferred time
+  // [4] work cursor, [5] earliest deferred time, [6] push records
   uint32_t *misc = reinterpret_cast<uint32_t *>(lds + a.o_misc);
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
@@ -672,7 +714,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     const uint32_t new32 = (ne + 31u) >> 5;  // element bitmask words
     if (tid == 0) a.act[entry] = 0u;
     if (a.wake[entry] > W) {
-      if (tid == 0) enqueue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
+      if (tid == 0) {
+        enqueue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
+        a.pcnt[entry] = 0u;
+      }
       continue;
     }
     for (uint32_t k = tid; k < new32; k += BS) {
@@ -684,6 +729,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
       misc[2] = 0u;
       misc[3] = 0u;
       misc[5] = __float_as_uint(F_INF);
+      misc[6] = 0u;
     }
     __syncthreads();
     // stage vertex times; dirty flags split into sources and deferred
@@ -714,6 +760,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
           tch[k + 1] = 0u;
           wrt[k] = 0u;
           wrt[k + 1] = 0u;
+        }
+        if (lane == 2 || lane == 3) {
+          const uint2 w = a.ifm[(size_t)s * a.wps + k + (lane - 2)];
+          ifc[k + lane - 2] = w.x;
+          ipre[k + lane - 2] = w.y;
         }
         wave_append(d, l, vlist, &misc[2]);
       }
@@ -836,20 +887,39 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     uint32_t cnt = 0;
     for (uint32_t k = tid; k < new32; k += BS) cnt += __popc(eseen[k]);
     if (cnt) atomicAdd(&misc[3], cnt);
-    // write back decreased copies; deferred vertices (and sources left by
-    // the sweep cap) stay dirty and the submesh is queued again
-    for (uint32_t l = tid; l < nn; l += BS) {
-      if (flag_at(tch, l)) {
-        a.phi[cb + l] = phi_l[l];
-        a.chg[cb + l] = 1;
+    // write back decreased copies, lowered interface copies as push records
+    // into the item's slot range (entry = interface copies before it);
+    // deferred vertices (and sources left by the sweep cap) stay dirty and
+    // the submesh is queued again
+    uint2 *rec = a.prec + (size_t)inst * a.n_if + si.ioff;
+    for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
+      const uint32_t l = l0 + tid;  // uniform trip count: ballots below
+      bool p = false;
+      if (l < nn) {
+        if (flag_at(tch, l)) {
+          a.phi[cb + l] = phi_l[l];
+          p = flag_at(ifc, l);
+        }
+        if (flag_at(dfr, l) || (capped && flag_at(flg, l))) {
+          a.dirty[cb + l] = 1;
+          atomicMin(&misc[5], __float_as_uint(phi_l[l]));
+        }
       }
-      if (flag_at(dfr, l) || (capped && flag_at(flg, l))) {
-        a.dirty[cb + l] = 1;
-        atomicMin(&misc[5], __float_as_uint(phi_l[l]));
+      const uint64_t m = __ballot(p);
+      if (m) {
+        uint32_t wb = 0;
+        if (lane == 0) wb = atomicAdd(&misc[6], (uint32_t)__popcll(m));
+        wb = __shfl(wb, 0);
+        if (p) {
+          const uint32_t w = l >> 5;
+          const uint32_t j = ipre[w] + __popc(ifc[w] & ((1u << (l & 31)) - 1u));
+          rec[wb + __popcll(m & lanes_below())] = make_uint2(si.ioff + j, __float_as_uint(phi_l[l]));
+        }
       }
     }
     __syncthreads();
     if (tid == 0) {
+      a.pcnt[entry] = misc[6];
       abytes += 48ull * misc[3] + 10ull * nn + 12ull * si.ni;
       const float wk = __uint_as_float(misc[5]);
       a.wake[entry] = wk;
@@ -870,52 +940,57 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 
 struct SyncArgs {
   const SubInfo *sub;
-  const uint16_t *if_l;
   const uint32_t *if_dsp;
   const uint32_t *if_other;
   const uint32_t *c2s;
+  const uint2 *prec;
+  const uint32_t *pcnt;
   float *phi;
   uint8_t *dirty;
-  uint8_t *chg;
   uint32_t *act;
   float *wake;
   WorkList cur, nxt;
   uint32_t *iter;
-  uint32_t nsub, ncopy, ninst, unit, maxch;
+  uint32_t nsub, ncopy, ninst, n_if, unit, maxch;
 };
 
-// interface push: min over copies, order independent
+// interface push: one wave per work unit of the iteration's list, a lane per
+// push record of its items; min over copies, order independent
 __global__ __launch_bounds__(256) void k_conv_sync(SyncArgs a) {
   // the compute kernel of this iteration has finished reading the counter
   if (blockIdx.x == 0 && threadIdx.x == 0) *a.iter += 1u;
-  // units of this workgroup's XCD eighth, each once (the grid has >= 8
-  // workgroups, so every eighth is covered)
-  const UnitRange R = xcd_range(a.cur.cnt[0], blockIdx.x % kXcd, gridDim.x);
-  for (uint32_t u = R.b + blockIdx.x / kXcd; u < R.e; u += R.nb) {
+  const uint32_t lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
+  const uint32_t U = a.cur.cnt[0], nw = gridDim.x * wpb;
+  for (uint32_t u = blockIdx.x * wpb + (threadIdx.x >> 6); u < U; u += nw) {
     const uint32_t pu = a.cur.plan[u];
     const uint32_t s = pu / a.maxch, ch = pu - s * a.maxch;
     const uint32_t n_in = min(a.unit, a.cur.bcnt[s] - ch * a.unit);
-    const SubInfo si = a.sub[s];
+    const uint32_t ioff = a.sub[s].ioff;
     for (uint32_t q = 0; q < n_in; ++q) {
       const uint32_t inst = a.cur.bins[(size_t)s * a.ninst + ch * a.unit + q];
+      const uint32_t n = a.pcnt[(size_t)inst * a.nsub + s];
+      const uint2 *rec = a.prec + (size_t)inst * a.n_if + ioff;
       const size_t ib = (size_t)inst * a.ncopy;
-      for (uint32_t j = threadIdx.x; j < si.ni; j += blockDim.x) {
-        const uint32_t idx = si.ioff + j;
-        const size_t gc = ib + si.coff + a.if_l[idx];
-        if (!a.chg[gc]) continue;
-        a.chg[gc] = 0;
-        const uint32_t v = __float_as_uint(a.phi[gc]);
-        for (uint32_t k = a.if_dsp[idx]; k < a.if_dsp[idx + 1]; ++k) {
+      for (uint32_t r = lane; r < n; r += 64) {
+        const uint2 e = rec[r];
+        const uint32_t v = e.y;
+        for (uint32_t k = a.if_dsp[e.x]; k < a.if_dsp[e.x + 1]; ++k) {
           const uint32_t oc = a.if_other[k];
           uint32_t *p = reinterpret_cast<uint32_t *>(a.phi + ib + oc);
+          // Only pushes (no compute kernel runs now) lower phi, so the
+          // smallest push to a copy always sees v below what it reads; the
+          // flags it sets (dirty, wake, queued) are therefore the same
+          // whatever the order, and the minima need no return value.
           if (v < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            const uint32_t old = atomicMin(p, v);
-            if (v < old) {
-              a.dirty[ib + oc] = 1;
-              const uint32_t s2 = a.c2s[oc];
-              atomicMin(reinterpret_cast<uint32_t *>(a.wake + (size_t)inst * a.nsub + s2), v);
+            __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            a.dirty[ib + oc] = 1;
+            const uint32_t s2 = a.c2s[oc];
+            const size_t pe = (size_t)inst * a.nsub + s2;
+            uint32_t *wk = reinterpret_cast<uint32_t *>(a.wake + pe);
+            if (v < __hip_atomic_load(wk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+              __hip_atomic_fetch_min(wk, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_load(&a.act[pe], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
               enqueue(a.nxt, a.act, inst, s2, a.nsub, a.ninst);
-            }
           }
         }
       }
@@ -1325,8 +1400,12 @@ struct fim_ctx {
   uint32_t *d_adj_ptr = nullptr;
   // converge state
   float *d_phi = nullptr, *d_rscale = nullptr;
-  uint8_t *d_dirty = nullptr, *d_chg = nullptr;
+  uint8_t *d_dirty = nullptr;
   uint32_t *d_act = nullptr, *d_cnt = nullptr;
+  uint2 *d_prec = nullptr;  // push records: n_inst x (interface entries)
+  uint32_t *d_pcnt = nullptr;
+  uint2 *d_ifm = nullptr;
+  uint32_t wps = 1;
   WorkList wl[3] = {};
   uint32_t unit = 2, maxch = 1;
   float *d_wake = nullptr;
@@ -1593,7 +1672,9 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         const uint32_t o_vlist = o_list + al16(2 * (uint32_t)c->ne_max);
         const uint32_t o_misc = o_vlist + (o->staged ? 0u : al16(2 * nn64));
         const uint32_t o_wrt = o_misc + 32;  // misc: 8 words
-        c->lds_conv = o_wrt + (o->staged ? 0u : al16(4 * (nn64 / 32)));
+        // unstaged: interface-copy bits and entry counts (2 x nn64/32 words)
+        const uint32_t o_ifc = o_wrt + (o->staged ? 0u : al16(4 * (nn64 / 32)));
+        c->lds_conv = o_ifc + (o->staged ? 0u : 2u * al16(4 * (nn64 / 32)));
         if (c->lds_conv > 160 * 1024)
           throw Error(FIM_ELIMIT, "largest submesh needs " + std::to_string(c->lds_conv) +
                                       " B of LDS (limit 163840); use a smaller partition budget");
@@ -1644,7 +1725,29 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         const size_t ncs = (size_t)c->ninst * c->ncopy, nss = (size_t)c->ninst * c->nsub;
         c->d_phi = dalloc<float>(ncs, ow);
         c->d_dirty = dalloc<uint8_t>(ncs, ow);
-        c->d_chg = dalloc<uint8_t>(ncs, ow);
+        {
+          // push records: one slot per (instance, interface entry)
+          c->d_prec = dalloc<uint2>((size_t)c->ninst * p.if_l.size(), ow);
+          c->d_pcnt = dalloc<uint32_t>(nss, ow);
+          // per submesh: interface-copy bits of each 32 local copies and the
+          // number of interface entries before that word (if_l ascends)
+          c->wps = (uint32_t)((c->nn_max + 63) / 64 * 2);
+          std::vector<uint2> ifm((size_t)c->nsub * c->wps, make_uint2(0u, 0u));
+          for (uint32_t s = 0; s < c->nsub; ++s) {
+            const SubInfo &si = p.sub[s];
+            for (uint32_t j = 0; j < si.ni; ++j) {
+              const uint32_t l = p.if_l[si.ioff + j];
+              if (j > 0 && l <= p.if_l[si.ioff + j - 1]) throw Error(FIM_EINVAL, "interface list not ascending");
+              ifm[(size_t)s * c->wps + (l >> 5)].x |= 1u << (l & 31);
+            }
+            uint32_t run = 0;
+            for (uint32_t w = 0; w < c->wps; ++w) {
+              ifm[(size_t)s * c->wps + w].y = run;
+              run += (uint32_t)__builtin_popcount(ifm[(size_t)s * c->wps + w].x);
+            }
+          }
+          c->d_ifm = dupload(ifm, ow);
+        }
         c->d_act = dalloc<uint32_t>(nss, ow);
         c->d_wake = dalloc<float>(nss, ow);
         c->unit = o->unit > 0 ? (uint32_t)o->unit : 2u;
@@ -1674,15 +1777,17 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         c->grid = std::max(c->grid, (int)kXcd);  // every XCD eighth needs a workgroup
         c->lds_compat = 0;
         c->ca = ConvArgs{c->d_sub, c->d_e2n, c->d_geo32, c->d_hgt, c->d_ereg, c->d_rscale, c->d_phi,
-                         c->d_dirty, c->d_chg, c->d_act, c->d_wake, WorkList{}, WorkList{},
-                         c->d_stats, c->d_cnt + 48,
+                         c->d_dirty, c->d_act, c->d_wake, c->d_if_l, c->d_ifm, c->d_prec,
+                         c->d_pcnt, WorkList{}, WorkList{}, c->d_stats, c->d_cnt + 48,
                          c->d_n2e_dsp, c->d_n2e_le, c->d_erec, c->d_adj_ptr, c->d_adj,
                          c->nsub, c->ncopy, c->ninst, (float)(o->eps_rel > 0 ? o->eps_rel : 1e-6),
-                         (float)o->window, o->local_iters, c->unit, c->maxch, o_phin, o_flg, o_tch, o_dfr, o_e2n,
-                         o_geo, o_list, o_vlist, o_misc, o_wrt};
-        c->sa = SyncArgs{c->d_sub, c->d_if_l, c->d_if_dsp, c->d_if_other, c->d_c2s, c->d_phi,
-                         c->d_dirty, c->d_chg, c->d_act, c->d_wake, WorkList{}, WorkList{},
-                         c->d_cnt + 48, c->nsub, c->ncopy, c->ninst, c->unit, c->maxch};
+                         (float)o->window, o->local_iters, c->unit, c->maxch, (uint32_t)p.if_l.size(),
+                         c->wps, o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc,
+                         o_wrt, o_ifc};
+        c->sa = SyncArgs{c->d_sub, c->d_if_dsp, c->d_if_other, c->d_c2s, c->d_prec, c->d_pcnt,
+                         c->d_phi, c->d_dirty, c->d_act, c->d_wake, WorkList{}, WorkList{},
+                         c->d_cnt + 48, c->nsub, c->ncopy, c->ninst, (uint32_t)p.if_l.size(),
+                         c->unit, c->maxch};
       } else {
         const uint32_t nn = (uint32_t)c->nn_max;
         const uint32_t o_tphi = al16(8 * nn);
@@ -1737,7 +1842,6 @@ int fim_reset(fim_ctx *c) {
       const size_t ncs = (size_t)c->ninst * c->ncopy, nss = (size_t)c->ninst * c->nsub;
       k_fill<float><<<blocks_for(ncs, 256), 256, 0, c->stream>>>(c->d_phi, ncs, F_INF);
       k_fill<uint8_t><<<blocks_for(ncs, 256), 256, 0, c->stream>>>(c->d_dirty, ncs, (uint8_t)0);
-      k_fill<uint8_t><<<blocks_for(ncs, 256), 256, 0, c->stream>>>(c->d_chg, ncs, (uint8_t)0);
       k_fill<uint32_t><<<blocks_for(nss, 256), 256, 0, c->stream>>>(c->d_act, nss, 0u);
       k_fill<float><<<blocks_for(nss, 256), 256, 0, c->stream>>>(c->d_wake, nss, F_INF);
       k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_cnt, 64, 0u);
