@@ -678,9 +678,12 @@ __device__ inline void wave_append(bool pred, uint32_t l, uint16_t *list, uint32
 //   accept: each written vertex takes its new time; a significant decrease
 //     (reference criterion, cuda_fim.cu:2154) makes it a source of the next
 //     sweep, or a deferred vertex if it lies beyond the window.
-// Element data never enters LDS.
-template <int BS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_conv_band(ConvArgs a) {
+// Element data never enters LDS.  WPE is the number of waves per SIMD the
+// register budget is sized for (512 / WPE VGPRs): the launch picks the one
+// that the LDS carve allows anyway, so registers left over by a large
+// submesh's LDS footprint are used instead of spilled.
+template <int BS, int WPE>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_conv_band(ConvArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   float *phi_l = reinterpret_cast<float *>(lds);
   uint32_t *phi_n = reinterpret_cast<uint32_t *>(lds + a.o_phin);
@@ -1429,6 +1432,7 @@ struct fim_ctx {
   CompatArgs xa{};
   uint32_t lds_conv = 0, lds_compat = 0;
   int grid = 0, block = 256, graph_iters = 8;
+  int band_wpe = 8;  // waves per SIMD the band kernel's registers are sized for
   fim_stats_t stats{};
 };
 
@@ -1444,26 +1448,37 @@ void launch_conv_compute(fim_ctx *c, const ConvArgs &a) {
     }
   } else {
     switch (c->block) {
-      case 64: k_conv_band<64><<<c->grid, 64, c->lds_conv, c->stream>>>(a); break;
-      case 128: k_conv_band<128><<<c->grid, 128, c->lds_conv, c->stream>>>(a); break;
-      case 512: k_conv_band<512><<<c->grid, 512, c->lds_conv, c->stream>>>(a); break;
-      default: k_conv_band<256><<<c->grid, 256, c->lds_conv, c->stream>>>(a); break;
+      case 64: k_conv_band<64, 8><<<c->grid, 64, c->lds_conv, c->stream>>>(a); break;
+      case 128: k_conv_band<128, 8><<<c->grid, 128, c->lds_conv, c->stream>>>(a); break;
+      case 512: k_conv_band<512, 8><<<c->grid, 512, c->lds_conv, c->stream>>>(a); break;
+      default:
+        if (c->band_wpe == 6)
+          k_conv_band<256, 6><<<c->grid, 256, c->lds_conv, c->stream>>>(a);
+        else if (c->band_wpe == 7)
+          k_conv_band<256, 7><<<c->grid, 256, c->lds_conv, c->stream>>>(a);
+        else
+          k_conv_band<256, 8><<<c->grid, 256, c->lds_conv, c->stream>>>(a);
+        break;
     }
   }
   HIPCHECK(hipGetLastError());
 }
 
-const void *conv_kernel_ptr(bool staged, int block) {
+const void *conv_kernel_ptr(bool staged, int block, int wpe) {
   switch (block) {
-    case 64: return staged ? (const void *)k_conv_compute<64, true> : (const void *)k_conv_band<64>;
-    case 128: return staged ? (const void *)k_conv_compute<128, true> : (const void *)k_conv_band<128>;
-    case 512: return staged ? (const void *)k_conv_compute<512, true> : (const void *)k_conv_band<512>;
-    default: return staged ? (const void *)k_conv_compute<256, true> : (const void *)k_conv_band<256>;
+    case 64: return staged ? (const void *)k_conv_compute<64, true> : (const void *)k_conv_band<64, 8>;
+    case 128: return staged ? (const void *)k_conv_compute<128, true> : (const void *)k_conv_band<128, 8>;
+    case 512: return staged ? (const void *)k_conv_compute<512, true> : (const void *)k_conv_band<512, 8>;
+    default:
+      if (staged) return (const void *)k_conv_compute<256, true>;
+      if (wpe == 6) return (const void *)k_conv_band<256, 6>;
+      if (wpe == 7) return (const void *)k_conv_band<256, 7>;
+      return (const void *)k_conv_band<256, 8>;
   }
 }
 
 void set_lds_attr(fim_ctx *c) {
-  const void *f = conv_kernel_ptr(c->o.staged != 0, c->block);
+  const void *f = conv_kernel_ptr(c->o.staged != 0, c->block, c->band_wpe);
   HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_conv));
   const void *g = nullptr;
   switch (c->block) {
@@ -1767,11 +1782,18 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         k_fill<float><<<blocks_for(4 * (size_t)c->ninst, 256), 256, 0, c->stream>>>(c->d_rscale, 4 * (size_t)c->ninst, 1.0f);
         int ncu = 256;
         HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
-        // one resident wave of workgroups: registers, LDS and wave slots
+        // one resident wave of workgroups: registers, LDS and wave slots.
+        // Workgroups of 256 put one wave on each SIMD, so the LDS carve
+        // alone caps the waves per SIMD at 160 KB / carve; the band kernel
+        // variant sized for that many waves gets the registers left over.
+        {
+          const uint32_t lds_cap = 160u * 1024u / std::max<uint32_t>(c->lds_conv, 1u);
+          c->band_wpe = lds_cap <= 6u ? 6 : (lds_cap == 7u ? 7 : 8);
+        }
         set_lds_attr(c);
         int per_cu = 0;
         HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, conv_kernel_ptr(o->staged != 0, c->block), c->block, c->lds_conv));
+            &per_cu, conv_kernel_ptr(o->staged != 0, c->block, c->band_wpe), c->block, c->lds_conv));
         per_cu = std::max(per_cu, 1);
         c->grid = o->grid > 0 ? o->grid : ncu * per_cu;
         c->grid = std::max(c->grid, (int)kXcd);  // every XCD eighth needs a workgroup
