@@ -191,7 +191,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--max-nn", type=int, default=1280)
+    ap.add_argument("--max-nn", type=int, default=1408)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--local-iters", type=int, default=4,
                     help="sweeps per activation before requeueing (0 = to local convergence)")
@@ -210,6 +210,8 @@ def main():
     ap.add_argument("--plain-max-nn", type=int, default=256)
     ap.add_argument("--plain-window", type=float, default=2000.0)
     ap.add_argument("--plain-local-iters", type=int, default=4)
+    ap.add_argument("--plain-waves", type=int, default=5,
+                    help="waves per SIMD of the plain solve's compute kernel (0 = LDS bound)")
     ap.add_argument("--nn", action="store_true",
                     help="also measure the surrogate warm start (adds about a minute)")
     ap.add_argument("--nn-maps", type=int, default=4096)
@@ -292,7 +294,7 @@ def main():
     ppack = pack if args.plain_max_nn == args.max_nn else \
         F.Pack(mesh, F.PART_RCB, max_nn=args.plain_max_nn, seed=args.seed)
     p1 = F.Solver(ppack, n_instances=1, block=args.block, profile=True, window=args.plain_window,
-                  local_iters=args.plain_local_iters,
+                  local_iters=args.plain_local_iters, waves=args.plain_waves,
                   staged=args.staged)
     solve_batch(p1, sets[:1])
     account(p1.stats())
@@ -368,7 +370,7 @@ def main():
         "graph_maps_per_s": graph_maps_per_s,
         "plain_maps_per_s": 1.0 / plain_s,
         "plain_config": {"max_nn": args.plain_max_nn, "window_us": args.plain_window,
-                         "local_iters": args.plain_local_iters},
+                         "local_iters": args.plain_local_iters, "waves": args.plain_waves},
         "batched_over_plain": maps_per_s * plain_s,
         "solves_per_s": maps_per_s,
         "elem_evals_per_s": tot["elem_updates"] / elapsed,

@@ -1452,7 +1452,9 @@ void launch_conv_compute(fim_ctx *c, const ConvArgs &a) {
       case 128: k_conv_band<128, 8><<<c->grid, 128, c->lds_conv, c->stream>>>(a); break;
       case 512: k_conv_band<512, 8><<<c->grid, 512, c->lds_conv, c->stream>>>(a); break;
       default:
-        if (c->band_wpe == 6)
+        if (c->band_wpe == 5)
+          k_conv_band<256, 5><<<c->grid, 256, c->lds_conv, c->stream>>>(a);
+        else if (c->band_wpe == 6)
           k_conv_band<256, 6><<<c->grid, 256, c->lds_conv, c->stream>>>(a);
         else if (c->band_wpe == 7)
           k_conv_band<256, 7><<<c->grid, 256, c->lds_conv, c->stream>>>(a);
@@ -1471,6 +1473,7 @@ const void *conv_kernel_ptr(bool staged, int block, int wpe) {
     case 512: return staged ? (const void *)k_conv_compute<512, true> : (const void *)k_conv_band<512, 8>;
     default:
       if (staged) return (const void *)k_conv_compute<256, true>;
+      if (wpe == 5) return (const void *)k_conv_band<256, 5>;
       if (wpe == 6) return (const void *)k_conv_band<256, 6>;
       if (wpe == 7) return (const void *)k_conv_band<256, 7>;
       return (const void *)k_conv_band<256, 8>;
@@ -1640,6 +1643,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
       c->block = o->block ? o->block : 256;
       if (c->block != 64 && c->block != 128 && c->block != 256 && c->block != 512)
         throw Error(FIM_EINVAL, "block must be 64, 128, 256 or 512");
+      if (o->waves < 0 || o->waves > 8) throw Error(FIM_EINVAL, "waves must be 0..8");
       c->ninst = o->mode == FIM_MODE_CONVERGE ? (uint32_t)std::max(1, o->n_instances) : 1u;
       // a multiple of 3 so every graph launch starts on list 0
       c->graph_iters = o->graph_iters > 0 ? 3 * ((o->graph_iters + 2) / 3) : 6;
@@ -1788,7 +1792,8 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         // variant sized for that many waves gets the registers left over.
         {
           const uint32_t lds_cap = 160u * 1024u / std::max<uint32_t>(c->lds_conv, 1u);
-          c->band_wpe = lds_cap <= 6u ? 6 : (lds_cap == 7u ? 7 : 8);
+          c->band_wpe = lds_cap <= 5u ? 5 : (lds_cap >= 8u ? 8 : (int)lds_cap);
+          if (o->waves > 0) c->band_wpe = std::min(c->band_wpe, std::max(o->waves, 5));
         }
         set_lds_attr(c);
         int per_cu = 0;

@@ -56,7 +56,7 @@ class Opts(C.Structure):
                 ("device", C.c_int32), ("block", C.c_int32), ("local_iters", C.c_int32),
                 ("graph_iters", C.c_int32), ("grid", C.c_int32), ("eps_rel", C.c_double),
                 ("apd", C.c_double), ("profile", C.c_int32), ("staged", C.c_int32),
-                ("window", C.c_double), ("unit", C.c_int32)]
+                ("window", C.c_double), ("unit", C.c_int32), ("waves", C.c_int32)]
 
 
 class NNOpts(C.Structure):
@@ -284,12 +284,12 @@ class Pack:
 class Solver:
     def __init__(self, pack: Pack, mode=MODE_CONVERGE, local=LOCAL_EXACT, n_instances=1, device=0,
                  block=256, local_iters=0, graph_iters=0, grid=0, eps_rel=0.0, apd=0.0,
-                 profile=False, window=0.0, staged=False, unit=0):
+                 profile=False, window=0.0, staged=False, unit=0, waves=0):
         self.pack = pack
         self.nn = pack.info["nn_global"]
         self.n_instances = n_instances
         o = Opts(mode, local, n_instances, device, block, local_iters, graph_iters, grid, eps_rel, apd,
-                 int(profile), int(staged), window, unit)
+                 int(profile), int(staged), window, unit, waves)
         self.h = _vp()
         _check(lib().fim_create(pack.h, C.byref(o), C.byref(self.h)))
 

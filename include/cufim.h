@@ -209,6 +209,11 @@ typedef struct {
                             one workgroup runs back to back as a work unit
                             (0 = 2); the units of a list are dealt out per
                             XCD so a submesh's element data stays in one L2 */
+  int32_t waves;         /* converge, workgroup 256: waves per SIMD the
+                            compute kernel's registers are sized for, 5..8
+                            (0 = as many as the LDS carve admits).  Fewer,
+                            spill-free waves suit lists too short to fill
+                            the chip (single solves).                       */
 } fim_opts;
 
 typedef struct {
@@ -218,8 +223,9 @@ typedef struct {
   int64_t elem_updates;     /* element evaluations                          */
   double last_ms;           /* device time of the last fim_run / fim_step   */
   int64_t alg_bytes;        /* algorithmic HBM bytes of the activations:
-                               32 per element + 10 per vertex copy + 12 per
-                               interface copy of each activated submesh     */
+                               48 per distinct element evaluated + 10 per
+                               vertex copy + 12 per interface copy of each
+                               activated submesh                            */
   int64_t compute_launches; /* compute-kernel launches timed (profile)      */
   double compute_ms;        /* summed compute-kernel time (profile)         */
   int64_t vertex_updates;   /* (element, target vertex) candidates computed */

@@ -137,6 +137,30 @@ def test_converge_workgroup_shapes_agree(block, staged):
     _check_fp32(phi, ref)
 
 
+@pytest.mark.parametrize("max_nn", [256, 1280])
+def test_converge_register_variants_agree(max_nn):
+    # the compute kernel built for 5..8 waves per SIMD: same arithmetic, so the
+    # same fixed point bit for bit, single and batched, and the FP64 oracle's
+    cfg = cases.scaled(cases.SLAB, 8)
+    m, om = cases.box(cfg, seed=11)
+    p = F.Pack(m, F.PART_RCB, max_nn=max_nn, seed=11)
+    srcs = [cases.pacing_sites(cfg, k + 1, seed=40 + k) for k in range(3)]
+    ref, _ = O.fim_solve(om, srcs[0], variant=O.LOCAL_EXACT)
+    base = None
+    for waves in (0, 5, 6, 7, 8):
+        s = F.Solver(p, n_instances=3, window=50.0, local_iters=4, waves=waves)
+        for i, src in enumerate(srcs):
+            s.seed(src, 0.0, inst=i)
+        s.run()
+        maps = [s.download(inst=i)[0] for i in range(3)]
+        s.close()
+        _check_fp32(maps[0], ref)
+        if base is None:
+            base = maps
+        for a_, b_ in zip(base, maps):
+            assert np.array_equal(a_, b_), waves
+
+
 def test_converge_deterministic_and_batched():
     cfg = cases.scaled(cases.SLAB, 8)
     m, om = cases.box(cfg, seed=7)

@@ -31,6 +31,21 @@ def test_library_exports_every_declared_symbol():
     assert names == set(F.SIGNATURES), names ^ set(F.SIGNATURES)
 
 
+def test_option_structs_match_the_header(tmp_path):
+    # the ctypes mirrors of the option structs have the C layout
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "cufim.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu\\n", sizeof(fim_opts),'
+                   ' offsetof(fim_opts, window), offsetof(fim_opts, waves),'
+                   ' sizeof(fim_part_opts), sizeof(fim_nn_opts));return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    assert got == [ctypes.sizeof(F.Opts), F.Opts.window.offset, F.Opts.waves.offset,
+                   ctypes.sizeof(F.PartOpts), ctypes.sizeof(F.NNOpts)]
+
+
 @need_ref
 def test_load_vtk_matches_reference():
     m = F.Mesh.load_vtk(UTV)
