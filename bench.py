@@ -55,8 +55,7 @@ def pacing_sets(cfg, batch, seed):
 
 def solve_batch(s, sets):
     s.reset()
-    for i, src in enumerate(sets):
-        s.seed(src, 0.0, inst=i)
+    s.seed_batch(sets, 0.0)
     return s.run(max_iters=1_000_000)
 
 

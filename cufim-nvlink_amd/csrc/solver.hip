@@ -1001,14 +1001,13 @@ __global__ __launch_bounds__(256) void k_conv_sync(SyncArgs a) {
   }
 }
 
-// seeds: copies (within one instance) get min(phi, t0), are flagged, and
-// their submesh is queued on the current list
-__global__ void k_conv_seed(const uint32_t *copies, uint32_t n, uint32_t inst, float t0,
-                            const uint32_t *c2s, float *phi, uint8_t *dirty, uint32_t *act,
-                            float *wake, WorkList l0, uint32_t nsub, uint32_t ncopy,
-                            uint32_t ninst) {
+// seeds: (instance, copy) pairs get min(phi, t0), are flagged, and their
+// submesh is queued on the current list
+__global__ void k_conv_seed(const uint2 *ic, uint32_t n, float t0, const uint32_t *c2s, float *phi,
+                            uint8_t *dirty, uint32_t *act, float *wake, WorkList l0, uint32_t nsub,
+                            uint32_t ncopy, uint32_t ninst) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t c = copies[i];
+    const uint32_t inst = ic[i].x, c = ic[i].y;
     const size_t gc = (size_t)inst * ncopy + c;
     atomicMin(reinterpret_cast<uint32_t *>(phi + gc), __float_as_uint(t0));
     dirty[gc] = 1;
@@ -1330,10 +1329,10 @@ __global__ void k_compat_sync(const uint32_t *grp_v, uint32_t ngrp, const uint32
   }
 }
 
-__global__ void k_compat_seed(const uint32_t *copies, uint32_t n, double t0, double *phi, int8_t *st) {
+__global__ void k_compat_seed(const uint2 *ic, uint32_t n, double t0, double *phi, int8_t *st) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    phi[copies[i]] = t0;
-    st[copies[i]] = FIM_NARROW;
+    phi[ic[i].y] = t0;
+    st[ic[i].y] = FIM_NARROW;
   }
 }
 
@@ -1419,6 +1418,9 @@ struct fim_ctx {
   // shared
   unsigned long long *d_stats = nullptr;
   uint32_t *h_pin = nullptr;
+  // seed staging: pinned host and device buffers of (instance, copy) pairs
+  uint2 *h_seed = nullptr, *d_seed = nullptr;
+  size_t seed_cap = 0;
   float *d_tmpf = nullptr;
   double *d_tmpd = nullptr;
   int32_t *d_tmpi = nullptr;
@@ -1569,6 +1571,8 @@ void destroy_ctx(fim_ctx *c) {
     if (e) (void)hipEventDestroy(e);
   for (void *p : c->owned) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
+  if (c->h_seed) (void)hipHostFree(c->h_seed);
+  if (c->d_seed) (void)hipFree(c->d_seed);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1883,33 +1887,69 @@ int fim_reset(fim_ctx *c) {
   });
 }
 
+namespace {
+// seeds (instance, copy) pairs at t0 through the context's staging buffers
+// (grown, never shrunk); returns once the pinned buffer may be reused
+void seed_pairs(fim_ctx *c, const std::vector<uint2> &ic, double t0) {
+  if (ic.empty()) return;
+  if (c->o.mode == FIM_MODE_CONVERGE && !(t0 >= 0.0)) throw Error(FIM_EINVAL, "converge mode needs t0 >= 0");
+  if (ic.size() >= (1ull << 32)) throw Error(FIM_ELIMIT, "too many seed copies");
+  HIPCHECK(hipSetDevice(c->dev));
+  if (ic.size() > c->seed_cap) {
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (c->h_seed) HIPCHECK(hipHostFree(c->h_seed));
+    if (c->d_seed) HIPCHECK(hipFree(c->d_seed));
+    c->h_seed = nullptr;
+    c->d_seed = nullptr;
+    c->seed_cap = 0;
+    const size_t cap = std::max<size_t>(ic.size(), 1024);
+    HIPCHECK(hipHostMalloc((void **)&c->h_seed, cap * sizeof(uint2), hipHostMallocDefault));
+    HIPCHECK(hipMalloc(&c->d_seed, cap * sizeof(uint2)));
+    c->seed_cap = cap;
+  }
+  std::memcpy(c->h_seed, ic.data(), ic.size() * sizeof(uint2));
+  HIPCHECK(hipMemcpyAsync(c->d_seed, c->h_seed, ic.size() * sizeof(uint2), hipMemcpyHostToDevice, c->stream));
+  const int g = blocks_for(ic.size(), 256);
+  if (c->o.mode == FIM_MODE_CONVERGE) {
+    k_conv_seed<<<g, 256, 0, c->stream>>>(c->d_seed, (uint32_t)ic.size(), (float)t0, c->d_c2s, c->d_phi,
+                                          c->d_dirty, c->d_act, c->d_wake, c->wl[0], c->nsub,
+                                          c->ncopy, c->ninst);
+  } else {
+    k_compat_seed<<<g, 256, 0, c->stream>>>(c->d_seed, (uint32_t)ic.size(), t0, c->d_phi64, c->d_st);
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(c->stream));
+}
+
+void append_copies(const fim_ctx *c, int32_t inst, const int64_t *vtx, int64_t n, std::vector<uint2> &out) {
+  for (uint32_t cp : copies_of(c, vtx, n)) out.push_back(make_uint2((uint32_t)inst, cp));
+}
+}  // namespace
+
 int fim_seed(fim_ctx *c, int32_t inst, const int64_t *vtx, int64_t n, double t0) {
   return guarded([&] {
     if (!c || (!vtx && n > 0) || n < 0) throw Error(FIM_EINVAL, "bad seed arguments");
     check_inst(c, inst);
-    if (n == 0) return;
-    HIPCHECK(hipSetDevice(c->dev));
-    std::vector<uint32_t> cp = copies_of(c, vtx, n);
-    uint32_t *d = nullptr;
-    HIPCHECK(hipMalloc(&d, cp.size() * sizeof(uint32_t)));
-    try {
-      HIPCHECK(hipMemcpyAsync(d, cp.data(), cp.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
-      const int g = blocks_for(cp.size(), 256);
-      if (c->o.mode == FIM_MODE_CONVERGE) {
-        if (!(t0 >= 0.0)) throw Error(FIM_EINVAL, "converge mode needs t0 >= 0");
-        k_conv_seed<<<g, 256, 0, c->stream>>>(d, (uint32_t)cp.size(), (uint32_t)inst, (float)t0,
-                                              c->d_c2s, c->d_phi, c->d_dirty, c->d_act,
-                                              c->d_wake, c->wl[0], c->nsub, c->ncopy, c->ninst);
-      } else {
-        k_compat_seed<<<g, 256, 0, c->stream>>>(d, (uint32_t)cp.size(), t0, c->d_phi64, c->d_st);
-      }
-      HIPCHECK(hipGetLastError());
-      HIPCHECK(hipStreamSynchronize(c->stream));
-    } catch (...) {
-      (void)hipFree(d);
-      throw;
+    std::vector<uint2> ic;
+    append_copies(c, inst, vtx, n, ic);
+    seed_pairs(c, ic, t0);
+  });
+}
+
+int fim_seed_batch(fim_ctx *c, int32_t n_inst, const int32_t *inst, const int64_t *off,
+                   const int64_t *vtx, double t0) {
+  return guarded([&] {
+    if (!c || n_inst < 0 || (n_inst > 0 && (!inst || !off))) throw Error(FIM_EINVAL, "bad seed arguments");
+    if (n_inst == 0) return;
+    if (off[0] != 0) throw Error(FIM_EINVAL, "off[0] must be 0");
+    if (off[n_inst] > 0 && !vtx) throw Error(FIM_EINVAL, "bad seed arguments");
+    std::vector<uint2> ic;
+    for (int32_t i = 0; i < n_inst; ++i) {
+      check_inst(c, inst[i]);
+      if (off[i + 1] < off[i]) throw Error(FIM_EINVAL, "offsets must not decrease");
+      append_copies(c, inst[i], vtx + off[i], off[i + 1] - off[i], ic);
     }
-    HIPCHECK(hipFree(d));
+    seed_pairs(c, ic, t0);
   });
 }
 

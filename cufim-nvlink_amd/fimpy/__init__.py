@@ -110,6 +110,7 @@ SIGNATURES = {
     "fim_create": (C.c_int, [_vp, C.POINTER(Opts), C.POINTER(_vp)]),
     "fim_reset": (C.c_int, [_vp]),
     "fim_seed": (C.c_int, [_vp, C.c_int32, _pi64, C.c_int64, C.c_double]),
+    "fim_seed_batch": (C.c_int, [_vp, C.c_int32, _pi32, _pi64, _pi64, C.c_double]),
     "fim_set_scaling": (C.c_int, [_vp, C.c_int32, _pf]),
     "fim_init_field": (C.c_int, [_vp, C.c_int32, _pf]),
     "fim_step": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _pi32]),
@@ -299,6 +300,18 @@ class Solver:
     def seed(self, vtx, t0=0.0, inst=0):
         v = np.ascontiguousarray(vtx, dtype=np.int64).reshape(-1)
         _check(lib().fim_seed(self.h, inst, _p(v, _pi64), v.shape[0], t0))
+
+    def seed_batch(self, sets, t0=0.0, insts=None):
+        """Seeds instance insts[i] (default i) with the vertices sets[i], all
+        in one transfer and one launch."""
+        ins = np.arange(len(sets), dtype=np.int32) if insts is None else \
+            np.ascontiguousarray(insts, dtype=np.int32)
+        assert ins.shape[0] == len(sets)
+        off = np.zeros(len(sets) + 1, dtype=np.int64)
+        off[1:] = np.cumsum([len(np.asarray(v).reshape(-1)) for v in sets])
+        v = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.int64).reshape(-1) for x in sets])
+                                 if len(sets) else np.zeros(0, np.int64))
+        _check(lib().fim_seed_batch(self.h, len(sets), _p(ins, _pi32), _p(off, _pi64), _p(v, _pi64), t0))
 
     def set_scaling(self, scale, inst=0):
         s = np.ascontiguousarray(scale, dtype=np.float32)

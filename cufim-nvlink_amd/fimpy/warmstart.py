@@ -38,7 +38,7 @@ def cold_batch(solver, solves):
     solver.reset()
     for i, (sites, sc) in enumerate(solves):
         solver.set_scaling(sc, inst=i)
-        solver.seed(sites, 0.0, inst=i)
+    solver.seed_batch([s for s, _ in solves], 0.0)
     solver.run(max_iters=1_000_000)
     return solver.stats()
 

@@ -238,6 +238,12 @@ int fim_reset(fim_ctx *c);
 /* seed stimulus vertices (global ids) of one instance with time t0 on all
  * their copies; update_events_gpu (src/eikonal/cuda_fim.cu:632-683) */
 int fim_seed(fim_ctx *c, int32_t inst, const int64_t *vtx, int64_t n, double t0);
+/* fim_seed for many instances in one transfer and one launch: instance
+ * inst[i] gets vertices vtx[off[i] .. off[i+1]) (off has n_inst+1 entries,
+ * off[0] = 0).  No counterpart in the reference (one solve per process);
+ * for batched solves. */
+int fim_seed_batch(fim_ctx *c, int32_t n_inst, const int32_t *inst, const int64_t *off,
+                   const int64_t *vtx, double t0);
 /* converge mode: per-instance, per-region scaling of both speeds (4 regions) */
 int fim_set_scaling(fim_ctx *c, int32_t inst, const float scale[4]);
 /* converge mode: initial upper-bound field (global, nn values) with every

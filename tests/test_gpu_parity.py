@@ -161,6 +161,31 @@ def test_converge_register_variants_agree(max_nn):
             assert np.array_equal(a_, b_), waves
 
 
+def test_seed_batch_equals_per_instance_seeds():
+    cfg = cases.scaled(cases.SLAB, 8)
+    m, _ = cases.box(cfg, seed=13)
+    p = F.Pack(m, F.PART_RCB, max_nn=256, seed=13)
+    sets = [cases.pacing_sites(cfg, k % 3 + 1, seed=60 + k) for k in range(4)]
+    a = F.Solver(p, n_instances=5, window=50.0)
+    b = F.Solver(p, n_instances=5, window=50.0)
+    for i, src in enumerate(sets):
+        a.seed(src, 0.0, inst=i + 1)
+    b.seed_batch(sets + [np.zeros(0, np.int64)], 0.0, insts=[1, 2, 3, 4, 0])
+    a.run()
+    b.run()
+    for i in range(5):
+        pa, sa = a.download(inst=i)
+        pb, sb = b.download(inst=i)
+        assert np.array_equal(pa, pb) and np.array_equal(sa, sb), i
+    assert np.all(a.download(inst=0)[0] > 1e50)  # instance 0 was never seeded
+    with pytest.raises(F.FimError):
+        b.seed_batch([sets[0]], 0.0, insts=[5])  # instance out of range
+    with pytest.raises(F.FimError):
+        b.seed_batch([np.array([m.nn], np.int64)], 0.0)  # vertex out of range
+    a.close()
+    b.close()
+
+
 def test_converge_deterministic_and_batched():
     cfg = cases.scaled(cases.SLAB, 8)
     m, om = cases.box(cfg, seed=7)
