@@ -944,8 +944,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
 struct SyncArgs {
   const SubInfo *sub;
   const uint32_t *if_dsp;
-  const uint32_t *if_other;
-  const uint32_t *c2s;
+  const uint2 *if_os;  // per other copy: (copy, its submesh)
   const uint2 *prec;
   const uint32_t *pcnt;
   float *phi;
@@ -978,22 +977,24 @@ __global__ __launch_bounds__(256) void k_conv_sync(SyncArgs a) {
         const uint2 e = rec[r];
         const uint32_t v = e.y;
         for (uint32_t k = a.if_dsp[e.x]; k < a.if_dsp[e.x + 1]; ++k) {
-          const uint32_t oc = a.if_other[k];
-          uint32_t *p = reinterpret_cast<uint32_t *>(a.phi + ib + oc);
-          // Only pushes (no compute kernel runs now) lower phi, so the
-          // smallest push to a copy always sees v below what it reads; the
-          // flags it sets (dirty, wake, queued) are therefore the same
-          // whatever the order, and the minima need no return value.
-          if (v < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          const uint2 o = a.if_os[k];
+          const size_t pe = (size_t)inst * a.nsub + o.y;
+          uint32_t *p = reinterpret_cast<uint32_t *>(a.phi + ib + o.x);
+          uint32_t *wk = reinterpret_cast<uint32_t *>(a.wake + pe);
+          // Only pushes (no compute kernel runs now) lower phi and wake and
+          // only pushes set act, so a stale read errs on the safe side (an
+          // extra atomic); the smallest push to a copy always sees v below
+          // what it reads, so the flags it sets (dirty, wake, queued) are the
+          // same whatever the order, and the minima need no return value.
+          // The three reads are issued together.
+          const uint32_t cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const uint32_t wcur = __hip_atomic_load(wk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const uint32_t queued = __hip_atomic_load(&a.act[pe], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (v < cur) {
             __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            a.dirty[ib + oc] = 1;
-            const uint32_t s2 = a.c2s[oc];
-            const size_t pe = (size_t)inst * a.nsub + s2;
-            uint32_t *wk = reinterpret_cast<uint32_t *>(a.wake + pe);
-            if (v < __hip_atomic_load(wk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-              __hip_atomic_fetch_min(wk, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (__hip_atomic_load(&a.act[pe], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
-              enqueue(a.nxt, a.act, inst, s2, a.nsub, a.ninst);
+            a.dirty[ib + o.x] = 1;
+            if (v < wcur) __hip_atomic_fetch_min(wk, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (queued == 0u) enqueue(a.nxt, a.act, inst, o.y, a.nsub, a.ninst);
           }
         }
       }
@@ -1394,7 +1395,8 @@ struct fim_ctx {
   uint32_t *d_c2g = nullptr, *d_c2s = nullptr, *d_g2c_dsp = nullptr, *d_g2c = nullptr, *d_grp = nullptr;
   uint32_t ngrp = 0;
   uint16_t *d_if_l = nullptr;
-  uint32_t *d_if_dsp = nullptr, *d_if_other = nullptr;
+  uint32_t *d_if_dsp = nullptr;
+  uint2 *d_if_os = nullptr;
   uint32_t *d_n2e_dsp = nullptr, *d_n2n_dsp = nullptr;
   uint16_t *d_n2e_le = nullptr, *d_n2n_lv = nullptr;
   uint8_t *d_n2e_pos = nullptr;
@@ -1744,7 +1746,11 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         }
         c->d_if_l = dupload(p.if_l, ow);
         c->d_if_dsp = dupload(p.if_dsp, ow);
-        c->d_if_other = dupload(p.if_other, ow);
+        {
+          std::vector<uint2> os(p.if_other.size());
+          for (size_t k = 0; k < os.size(); ++k) os[k] = make_uint2(p.if_other[k], p.c2s[p.if_other[k]]);
+          c->d_if_os = dupload(os, ow);
+        }
         const size_t ncs = (size_t)c->ninst * c->ncopy, nss = (size_t)c->ninst * c->nsub;
         c->d_phi = dalloc<float>(ncs, ow);
         c->d_dirty = dalloc<uint8_t>(ncs, ow);
@@ -1815,7 +1821,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
                          (float)o->window, o->local_iters, c->unit, c->maxch, (uint32_t)p.if_l.size(),
                          c->wps, o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc,
                          o_wrt, o_ifc};
-        c->sa = SyncArgs{c->d_sub, c->d_if_dsp, c->d_if_other, c->d_c2s, c->d_prec, c->d_pcnt,
+        c->sa = SyncArgs{c->d_sub, c->d_if_dsp, c->d_if_os, c->d_prec, c->d_pcnt,
                          c->d_phi, c->d_dirty, c->d_act, c->d_wake, WorkList{}, WorkList{},
                          c->d_cnt + 48, c->nsub, c->ncopy, c->ninst, (uint32_t)p.if_l.size(),
                          c->unit, c->maxch};
