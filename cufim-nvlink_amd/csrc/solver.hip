@@ -92,6 +92,16 @@ __device__ inline void enqueue(const WorkList &l, uint32_t *act, uint32_t inst, 
   }
 }
 
+// A work item queues itself for the next iteration.  Its act flag was
+// cleared when the item started and only the item's own workgroup can set
+// it before the iteration's sync kernel runs, so no exchange is needed.
+__device__ inline void requeue(const WorkList &l, uint32_t *act, uint32_t inst, uint32_t s,
+                               uint32_t nsub, uint32_t ninst) {
+  act[(size_t)inst * nsub + s] = 1u;
+  const uint32_t j = atomicAdd(&l.bcnt[s], 1u);
+  l.bins[(size_t)s * ninst + j] = inst;
+}
+
 struct UnitRange {
   uint32_t b, e, nb;  // units [b, e); nb workgroups of the grid run on this XCD
 };
@@ -335,7 +345,7 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
     // time: keep it queued without staging anything
     if (a.wake[entry] > W) {
       if (tid == 0) {
-        enqueue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
+        requeue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
         a.pcnt[entry] = 0u;
       }
       continue;
@@ -588,7 +598,7 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
     if (tid == 0) {
       const float wk = __uint_as_float(misc[1]);
       a.wake[entry] = wk;
-      if (wk < F_INF) enqueue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
+      if (wk < F_INF) requeue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
     }
     __syncthreads();
   }
@@ -718,7 +728,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
     if (tid == 0) a.act[entry] = 0u;
     if (a.wake[entry] > W) {
       if (tid == 0) {
-        enqueue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
+        requeue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
         a.pcnt[entry] = 0u;
       }
       continue;
@@ -926,7 +936,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
       abytes += 48ull * misc[3] + 10ull * nn + 12ull * si.ni;
       const float wk = __uint_as_float(misc[5]);
       a.wake[entry] = wk;
-      if (wk < F_INF) enqueue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
+      if (wk < F_INF) requeue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
     }
     __syncthreads();
   }
