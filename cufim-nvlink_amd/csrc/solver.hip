@@ -629,30 +629,24 @@ __device__ inline void expand_bits(uint32_t bits, uint32_t k, uint16_t *list, ui
   }
 }
 
-// marks local element le for the next sweep; the first marker appends it to
-// the element list
-__device__ inline void mark_append(uint32_t *emark, uint16_t *elist, uint32_t *cnt, uint32_t le) {
-  const uint32_t bit = 1u << (le & 31);
-  if (!(atomicOr(&emark[le >> 5], bit) & bit)) elist[atomicAdd(cnt, 1u)] = (uint16_t)le;
-}
-
-__device__ inline void mark_word(uint32_t *emark, uint16_t *elist, uint32_t *cnt, uint32_t w) {
+// marks the (up to two) local elements of one adjacency word in the LDS
+// bitmask; the atomics return nothing, the list is built from the bitmask
+__device__ inline void mark_word(uint32_t *emark, uint32_t w) {
   const uint32_t lo = w & 0xffffu, hi = w >> 16;
-  if (lo != 0xffffu) mark_append(emark, elist, cnt, lo);
-  if (hi != 0xffffu) mark_append(emark, elist, cnt, hi);
+  if (lo != 0xffffu) atomicOr(&emark[lo >> 5], 1u << (lo & 31));
+  if (hi != 0xffffu) atomicOr(&emark[hi >> 5], 1u << (hi & 31));
 }
 
-__device__ inline void mark_chunk(uint32_t *emark, uint16_t *elist, uint32_t *cnt, const uint4 q) {
-  mark_word(emark, elist, cnt, q.x);
-  mark_word(emark, elist, cnt, q.y);
-  mark_word(emark, elist, cnt, q.z);
-  mark_word(emark, elist, cnt, q.w);
+__device__ inline void mark_chunk(uint32_t *emark, const uint4 q) {
+  mark_word(emark, q.x);
+  mark_word(emark, q.y);
+  mark_word(emark, q.z);
+  mark_word(emark, q.w);
 }
 
 // marks the incident elements of copy c (global copy id): up to four 8-entry
 // chunks of the padded adjacency are loaded before any is used
-__device__ inline void mark_incident(const ConvArgs &a, uint32_t c, uint32_t *emark, uint16_t *elist,
-                                     uint32_t *cnt) {
+__device__ inline void mark_incident(const ConvArgs &a, uint32_t c, uint32_t *emark) {
   const uint32_t ap = a.adj_ptr[c];
   const uint4 *ch = a.adj + (ap >> 6);
   const uint32_t nch = ap & 63u;
@@ -661,11 +655,11 @@ __device__ inline void mark_incident(const ConvArgs &a, uint32_t c, uint32_t *em
   const uint4 q1 = nch > 1 ? ch[1] : pad;
   const uint4 q2 = nch > 2 ? ch[2] : pad;
   const uint4 q3 = nch > 3 ? ch[3] : pad;
-  mark_chunk(emark, elist, cnt, q0);
-  mark_chunk(emark, elist, cnt, q1);
-  mark_chunk(emark, elist, cnt, q2);
-  mark_chunk(emark, elist, cnt, q3);
-  for (uint32_t k = 4; k < nch; ++k) mark_chunk(emark, elist, cnt, ch[k]);
+  mark_chunk(emark, q0);
+  mark_chunk(emark, q1);
+  mark_chunk(emark, q2);
+  mark_chunk(emark, q3);
+  for (uint32_t k = 4; k < nch; ++k) mark_chunk(emark, ch[k]);
 }
 
 // appends the lanes with `pred` (vertex l) to list[*cnt ...], one LDS atomic
@@ -681,8 +675,9 @@ __device__ inline void wave_append(bool pred, uint32_t l, uint16_t *list, uint32
 // Unstaged converge kernel.  Per sweep it touches only the band that is
 // active, in three steps separated by barriers:
 //   mark: the sources (a compact list) mark their incident elements through
-//     the vertex -> element adjacency; the first marker of an element
-//     appends it to the element list;
+//     the vertex -> element adjacency in an LDS bitmask (atomics without
+//     return), then one thread per bitmask word lists the marked elements in
+//     id order and clears the word;
 //   evaluate: the listed elements compute candidates from L2-resident
 //     element records; LDS atomicMin winners set a "written" bit;
 //   accept: each written vertex takes its new time; a significant decrease
@@ -795,8 +790,18 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
         break;
       }
       ++it;
-      // mark the elements incident to the sources
-      for (uint32_t i = tid; i < nsrc; i += BS) mark_incident(a, si.coff + vlist[i], emark, elist, &misc[0]);
+      // mark the elements incident to the sources, then list the marked
+      // elements in id order (clearing the marks, one word per thread)
+      for (uint32_t i = tid; i < nsrc; i += BS) mark_incident(a, si.coff + vlist[i], emark);
+      __syncthreads();
+      for (uint32_t k = tid; k < new32; k += BS) {
+        const uint32_t bits = emark[k];
+        if (bits) {
+          emark[k] = 0u;
+          eseen[k] |= bits;
+          expand_bits(bits, k, elist, atomicAdd(&misc[0], (uint32_t)__popc(bits)));
+        }
+      }
       __syncthreads();
       const uint32_t nact = misc[0];
       evals += nact;
@@ -804,9 +809,6 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
       // candidates of the listed elements
       for (uint32_t i = tid; i < nact; i += BS) {
         const uint32_t e = elist[i];
-        const uint32_t ebit = 1u << (e & 31);
-        atomicAnd(&emark[e >> 5], ~ebit);
-        atomicOr(&eseen[e >> 5], ebit);
         const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
         const uint4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
         const float sc = rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
