@@ -1,12 +1,14 @@
 """Autotuner of the converge-mode solver: picks the submesh size, the
-workgroup shape, the local sweeps per global iteration and the time window
-for a mesh and batch size by measuring short batched solves on the GPU.
+workgroup shape, the local sweeps per global iteration, the time window and
+the compute kernel's register budget for a mesh and batch size by measuring
+short batched solves on the GPU.
 
 Search: coordinate descent from the defaults over
-    max_nn      {256, 512, 1024, 2048}   vertex copies per submesh
-    block       {64, 128, 256}           workgroup size
-    local_iters {0, 1, 2, 4, 8}          sweeps per activation (0 = to local convergence)
-    window      {500, 1000, 2000, 4000}  microseconds per global iteration
+    max_nn      {256, 512, 1024, 1408, 2048}  vertex copies per submesh
+    block       {64, 128, 256}                workgroup size
+    local_iters {0, 1, 2, 4, 8}               sweeps per activation (0 = to local convergence)
+    window      {500, 1000, 2000, 4000}       microseconds per global iteration
+    waves       {0, 5}                        waves per SIMD of the compute kernel (0 = LDS bound)
 Each trial times one batch solve after one warm-up solve.  Results are cached
 in a JSON file keyed by a mesh signature (sizes, bounding box, coordinate
 checksum) and the batch size.
@@ -23,12 +25,13 @@ import numpy as np
 from . import PART_RCB, Mesh, Pack, Solver
 
 SPACE = {
-    "max_nn": [256, 512, 1024, 2048],
+    "max_nn": [256, 512, 1024, 1408, 2048],
     "block": [64, 128, 256],
     "local_iters": [0, 1, 2, 4, 8],
     "window": [500.0, 1000.0, 2000.0, 4000.0],
+    "waves": [0, 5],
 }
-DEFAULT = {"max_nn": 1024, "block": 256, "local_iters": 4, "window": 2000.0}
+DEFAULT = {"max_nn": 1408, "block": 256, "local_iters": 4, "window": 2000.0, "waves": 0}
 
 
 def signature(mesh: Mesh, n_instances: int) -> str:
@@ -47,14 +50,14 @@ def _measure(mesh, packs, cfg, seeds, scales, seed):
     if key not in packs:
         packs[key] = Pack(mesh, PART_RCB, max_nn=key, seed=seed)
     s = Solver(packs[key], n_instances=len(seeds), block=cfg["block"],
-               local_iters=cfg["local_iters"], window=cfg["window"])
+               local_iters=cfg["local_iters"], window=cfg["window"], waves=cfg.get("waves", 0))
     try:
         def solve():
             s.reset()
-            for i, src in enumerate(seeds):
-                s.seed(src, 0.0, inst=i)
-                if scales is not None:
+            if scales is not None:
+                for i in range(len(seeds)):
                     s.set_scaling(scales[i], inst=i)
+            s.seed_batch(seeds, 0.0)
             s.run(max_iters=1_000_000)
         solve()
         t0 = time.perf_counter()
@@ -91,7 +94,7 @@ def tune(mesh: Mesh, seeds, scales=None, space=None, cache=None, seed=0, rounds=
     best_v = run(best)
     for _ in range(rounds):
         improved = False
-        for k in ("window", "max_nn", "block", "local_iters"):
+        for k in [k for k in ("window", "max_nn", "block", "local_iters", "waves") if k in space]:
             for v in space[k]:
                 cand = dict(best, **{k: v})
                 r = run(cand)

@@ -21,7 +21,8 @@ def test_autotune_picks_measured_best_and_caches(tmp_path):
     solves = W.multi_solves(cfgs, offs, 16, 5)
     seeds = [s["sites"] for s in solves]
     scales = np.stack([s["scale"] for s in solves])
-    space = {"max_nn": [256, 512], "block": [128, 256], "local_iters": [0, 2], "window": [1000.0]}
+    space = {"max_nn": [256, 512], "block": [128, 256], "local_iters": [0, 2], "window": [1000.0],
+             "waves": [0, 5]}
     cache = str(tmp_path / "tune.json")
     best, trials = A.tune(m, seeds, scales, space=space, cache=cache, rounds=1)
     assert best["max_nn"] in space["max_nn"] and best["block"] in space["block"]
@@ -32,7 +33,7 @@ def test_autotune_picks_measured_best_and_caches(tmp_path):
     # the tuned configuration solves correctly (one instance vs the oracle)
     p = F.Pack(m, F.PART_RCB, max_nn=best["max_nn"], seed=0)
     s = F.Solver(p, n_instances=1, block=best["block"], local_iters=best["local_iters"],
-                 window=best["window"])
+                 window=best["window"], waves=best["waves"])
     s.seed(seeds[1], 0.0)
     s.set_scaling(scales[1])
     s.run()
