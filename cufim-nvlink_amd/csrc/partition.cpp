@@ -193,8 +193,10 @@ int64_t converge_lds_bytes(int32_t nn_max, int32_t ne_max) {
   auto al = [](int64_t x) { return (x + 15) & ~int64_t(15); };
   const int64_t nn64 = (nn_max + 63) & ~int64_t(63);
   const int64_t ne32 = (ne_max + 31) / 32;
+  // (the source list doubles as the (element, target) pair rings of the
+  // four waves of a workgroup of 256: at least 4 x 320 u16)
   return 2 * al(4 * nn64) + 4 * al(4 * (nn64 / 32)) + 2 * al(4 * ne32) + al(2LL * ne_max) +
-         al(2 * nn64) + 32 + 2 * al(4 * (nn64 / 32));
+         al(std::max<int64_t>(2 * nn64, ne_max < (1 << 14) ? 2560 : 0)) + 32 + 2 * al(4 * (nn64 / 32));
 }
 
 Pack build_pack(const Mesh &m, const Partition &p, int32_t source) {

@@ -619,6 +619,9 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
 //   [2] node heights h0..h3 (f32)
 // Local copy ids are below 2^15, so the two region bits are free.
 constexpr uint32_t kAdjPad = 0xffffffffu;
+// (element, target) pair ring of one wave in the band kernel: up to 63 pairs
+// left from earlier rounds plus 4 targets of 64 elements
+constexpr uint32_t kPairRing = 320;
 
 // expands the set bits of `bits` (word k of a bitmask) into list[base...]
 __device__ inline void expand_bits(uint32_t bits, uint32_t k, uint16_t *list, uint32_t base) {
@@ -683,11 +686,13 @@ __device__ inline void wave_append(bool pred, uint32_t l, uint16_t *list, uint32
 //   accept: each written vertex takes its new time; a significant decrease
 //     (reference criterion, cuda_fim.cu:2154) makes it a source of the next
 //     sweep, or a deferred vertex if it lies beyond the window.
-// Element data never enters LDS.  WPE is the number of waves per SIMD the
+// Element data never enters LDS.  PAIRS evaluates (element, target) pairs
+// in full rounds of 64 instead of an element's four targets per lane (local
+// element ids below 2^14).  WPE is the number of waves per SIMD the
 // register budget is sized for (512 / WPE VGPRs): the launch picks the one
 // that the LDS carve allows anyway, so registers left over by a large
 // submesh's LDS footprint are used instead of spilled.
-template <int BS, int WPE>
+template <int BS, int WPE, bool PAIRS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_conv_band(ConvArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   float *phi_l = reinterpret_cast<float *>(lds);
@@ -807,52 +812,159 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
       evals += nact;
       if (tid == 0) misc[2] = 0u;  // every thread has read the source count
       // candidates of the listed elements
-      for (uint32_t i = tid; i < nact; i += BS) {
-        const uint32_t e = elist[i];
-        const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
-        const uint4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-        const float sc = rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
-        const float g[6] = {__uint_as_float(r0.z) * sc, __uint_as_float(r0.w) * sc,
-                            __uint_as_float(r1.x) * sc, __uint_as_float(r1.y) * sc,
-                            __uint_as_float(r1.z) * sc, __uint_as_float(r1.w) * sc};
-        // square roots (edge candidates) and reciprocals (triangle bases) of
-        // the six squared lengths, shared by the element's four targets
-        float sq[6], rc[6];
+      if constexpr (PAIRS) {
+        // (element, target) pairs: a pass over 64 listed elements per wave
+        // keeps the targets that pass the bound test and queues them in a
+        // wave-private ring (LDS, in the source list's space, idle until the
+        // accept step); full rounds of 64 pairs then evaluate one target
+        // per lane, so no lane idles on a pruned target
+        uint16_t *pq = vlist + wave * kPairRing;
+        uint32_t qh = 0, qn = 0;  // ring head and fill, uniform across the wave
+        for (uint32_t c0 = wave * 64;; c0 += BS) {
+          const bool more = c0 < nact;  // uniform across the wave
+          if (more) {
+            const uint32_t i = c0 + lane;
+            uint32_t tm = 0, e = 0;
+            if (i < nact) {
+              e = elist[i];
+              const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
+              const uint4 r0 = rec[0], r2 = rec[2];
+              const float hs = csqrt(rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)]);
+              const float h[4] = {__uint_as_float(r2.x) * hs, __uint_as_float(r2.y) * hs,
+                                  __uint_as_float(r2.z) * hs, __uint_as_float(r2.w) * hs};
+              const uint32_t nd[4] = {r0.x & 0x7fffu, (r0.x >> 16) & 0x7fffu, r0.y & 0xffffu,
+                                      r0.y >> 16};
+              float ph[4];
+              int f[4];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          sq[k] = csqrt(g[k]);
-          rc[k] = __builtin_amdgcn_rcpf(g[k]);
-        }
-        const float hs = csqrt(sc);
-        const float h[4] = {__uint_as_float(r2.x) * hs, __uint_as_float(r2.y) * hs,
-                            __uint_as_float(r2.z) * hs, __uint_as_float(r2.w) * hs};
-        const uint32_t nd[4] = {r0.x & 0x7fffu, (r0.x >> 16) & 0x7fffu, r0.y & 0xffffu,
-                                r0.y >> 16};
-        float ph[4];
-        int f[4];
+              for (int k = 0; k < 4; ++k) {
+                ph[k] = phi_l[nd[k]];
+                f[k] = flag_at(flg, nd[k]) ? 1 : 0;
+              }
+              const int nf = f[0] + f[1] + f[2] + f[3];
+              // targets with a source among the other nodes whose bound
+              // (earliest other node + metric height) is still below their time
+              tm |= (nf - f[0] > 0 && ph[0] > fminf(ph[1], fminf(ph[2], ph[3])) + h[0]) ? 1u : 0u;
+              tm |= (nf - f[1] > 0 && ph[1] > fminf(ph[0], fminf(ph[2], ph[3])) + h[1]) ? 2u : 0u;
+              tm |= (nf - f[2] > 0 && ph[2] > fminf(ph[0], fminf(ph[1], ph[3])) + h[2]) ? 4u : 0u;
+              tm |= (nf - f[3] > 0 && ph[3] > fminf(ph[0], fminf(ph[1], ph[2])) + h[3]) ? 8u : 0u;
+            }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          ph[k] = phi_l[nd[k]];
-          f[k] = flag_at(flg, nd[k]) ? 1 : 0;
+            for (uint32_t T = 0; T < 4; ++T) {
+              const bool p = (tm >> T) & 1u;
+              const uint64_t m = __ballot(p);
+              if (p) {
+                uint32_t pos = qh + qn + (uint32_t)__popcll(m & lanes_below());
+                if (pos >= kPairRing) pos -= kPairRing;
+                pq[pos] = (uint16_t)(e | (T << 14));
+              }
+              qn += (uint32_t)__popcll(m);
+            }
+            __builtin_amdgcn_wave_barrier();
+          }
+          while (qn >= 64u || (!more && qn > 0u)) {
+            const uint32_t take = qn < 64u ? qn : 64u;
+            if (lane < take) {
+              uint32_t pos = qh + lane;
+              if (pos >= kPairRing) pos -= kPairRing;
+              const uint32_t code = pq[pos];
+              const uint32_t e = code & 0x3fffu, T = code >> 14;
+              const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
+              const uint4 r0 = rec[0], r1 = rec[1];
+              const float sc = rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
+              const float g[6] = {__uint_as_float(r0.z) * sc, __uint_as_float(r0.w) * sc,
+                                  __uint_as_float(r1.x) * sc, __uint_as_float(r1.y) * sc,
+                                  __uint_as_float(r1.z) * sc, __uint_as_float(r1.w) * sc};
+              const uint32_t nd[4] = {r0.x & 0x7fffu, (r0.x >> 16) & 0x7fffu, r0.y & 0xffffu,
+                                      r0.y >> 16};
+              // relabel so that the target is node 3 (others T+1, T+2, T+3):
+              // the same candidate as for node position T
+              const bool t1 = T & 1u, t2 = T & 2u;
+              auto sel = [&](float x0, float x1, float x2, float x3) {
+                return t2 ? (t1 ? x3 : x2) : (t1 ? x1 : x0);
+              };
+              auto seli = [&](uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
+                return t2 ? (t1 ? x3 : x2) : (t1 ? x1 : x0);
+              };
+              // edge (i,j) of the relabelled element for T = 0, 1, 2, 3
+              const float gp[6] = {sel(g[2], g[5], g[3], g[0]), sel(g[4], g[1], g[4], g[1]),
+                                   sel(g[5], g[3], g[0], g[2]), sel(g[0], g[2], g[5], g[3]),
+                                   sel(g[1], g[4], g[1], g[4]), sel(g[3], g[0], g[2], g[5])};
+              const uint32_t n1 = seli(nd[1], nd[2], nd[3], nd[0]);
+              const uint32_t n2 = seli(nd[2], nd[3], nd[0], nd[1]);
+              const uint32_t n3 = seli(nd[3], nd[0], nd[1], nd[2]);
+              const uint32_t nt = seli(nd[0], nd[1], nd[2], nd[3]);
+              const float pp[4] = {phi_l[n1], phi_l[n2], phi_l[n3], phi_l[nt]};
+              float sq[6], rc[6];
+#pragma unroll
+              for (int k = 0; k < 3; ++k) {
+                rc[k] = __builtin_amdgcn_rcpf(gp[k]);
+                sq[k + 3] = csqrt(gp[k + 3]);
+                sq[k] = 0.0f;
+                rc[k + 3] = 0.0f;
+              }
+              const float c = cand_exact_sh<3, float>(gp, sq, rc, pp, F_INF);
+              ++vupd;
+              if (c < pp[3]) {
+                const uint32_t cbits = __float_as_uint(c);
+                if (cbits < atomicMin(&phi_n[nt], cbits)) atomicOr(&wrt[nt >> 5], 1u << (nt & 31));
+              }
+            }
+            qh += take;
+            if (qh >= kPairRing) qh -= kPairRing;
+            qn -= take;
+            __builtin_amdgcn_wave_barrier();
+          }
+          if (!more) break;
         }
-        const int nf = f[0] + f[1] + f[2] + f[3];
-        // targets with a source among the other nodes whose bound (earliest
-        // other node + metric height) is still below their time
-#define FIM_TARGET(T, A, B, C)                                                        \
-        if (nf - f[T] > 0 && ph[T] > fminf(ph[A], fminf(ph[B], ph[C])) + h[T]) {         \
-          const float c = cand_exact_sh<T, float>(g, sq, rc, ph, F_INF);                 \
-          ++vupd;                                                                         \
-          if (c < ph[T]) {                                                                \
-            const uint32_t cbits = __float_as_uint(c);                                    \
-            if (cbits < atomicMin(&phi_n[nd[T]], cbits))                                  \
-              atomicOr(&wrt[nd[T] >> 5], 1u << (nd[T] & 31));                             \
-          }                                                                               \
+      } else {
+        for (uint32_t i = tid; i < nact; i += BS) {
+          const uint32_t e = elist[i];
+          const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
+          const uint4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+          const float sc = rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
+          const float g[6] = {__uint_as_float(r0.z) * sc, __uint_as_float(r0.w) * sc,
+                              __uint_as_float(r1.x) * sc, __uint_as_float(r1.y) * sc,
+                              __uint_as_float(r1.z) * sc, __uint_as_float(r1.w) * sc};
+          // square roots (edge candidates) and reciprocals (triangle bases) of
+          // the six squared lengths, shared by the element's four targets
+          float sq[6], rc[6];
+  #pragma unroll
+          for (int k = 0; k < 6; ++k) {
+            sq[k] = csqrt(g[k]);
+            rc[k] = __builtin_amdgcn_rcpf(g[k]);
+          }
+          const float hs = csqrt(sc);
+          const float h[4] = {__uint_as_float(r2.x) * hs, __uint_as_float(r2.y) * hs,
+                              __uint_as_float(r2.z) * hs, __uint_as_float(r2.w) * hs};
+          const uint32_t nd[4] = {r0.x & 0x7fffu, (r0.x >> 16) & 0x7fffu, r0.y & 0xffffu,
+                                  r0.y >> 16};
+          float ph[4];
+          int f[4];
+  #pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            ph[k] = phi_l[nd[k]];
+            f[k] = flag_at(flg, nd[k]) ? 1 : 0;
+          }
+          const int nf = f[0] + f[1] + f[2] + f[3];
+          // targets with a source among the other nodes whose bound (earliest
+          // other node + metric height) is still below their time
+  #define FIM_TARGET(T, A, B, C)                                                        \
+          if (nf - f[T] > 0 && ph[T] > fminf(ph[A], fminf(ph[B], ph[C])) + h[T]) {         \
+            const float c = cand_exact_sh<T, float>(g, sq, rc, ph, F_INF);                 \
+            ++vupd;                                                                         \
+            if (c < ph[T]) {                                                                \
+              const uint32_t cbits = __float_as_uint(c);                                    \
+              if (cbits < atomicMin(&phi_n[nd[T]], cbits))                                  \
+                atomicOr(&wrt[nd[T] >> 5], 1u << (nd[T] & 31));                             \
+            }                                                                               \
+          }
+          FIM_TARGET(0, 1, 2, 3)
+          FIM_TARGET(1, 0, 2, 3)
+          FIM_TARGET(2, 0, 1, 3)
+          FIM_TARGET(3, 0, 1, 2)
+  #undef FIM_TARGET
         }
-        FIM_TARGET(0, 1, 2, 3)
-        FIM_TARGET(1, 0, 2, 3)
-        FIM_TARGET(2, 0, 1, 3)
-        FIM_TARGET(3, 0, 1, 2)
-#undef FIM_TARGET
       }
       __syncthreads();
       if (tid == 0) misc[0] = 0u;  // every thread has read the element count
@@ -1449,10 +1561,13 @@ struct fim_ctx {
   uint32_t lds_conv = 0, lds_compat = 0;
   int grid = 0, block = 256, graph_iters = 8;
   int band_wpe = 8;  // waves per SIMD the band kernel's registers are sized for
+  bool band_pairs = false;  // band kernel evaluates (element, target) pairs
   fim_stats_t stats{};
 };
 
 namespace {
+
+const void *conv_kernel_ptr(bool staged, int block, int wpe, bool pairs);
 
 void launch_conv_compute(fim_ctx *c, const ConvArgs &a) {
   if (c->o.staged) {
@@ -1464,40 +1579,42 @@ void launch_conv_compute(fim_ctx *c, const ConvArgs &a) {
     }
   } else {
     switch (c->block) {
-      case 64: k_conv_band<64, 8><<<c->grid, 64, c->lds_conv, c->stream>>>(a); break;
-      case 128: k_conv_band<128, 8><<<c->grid, 128, c->lds_conv, c->stream>>>(a); break;
-      case 512: k_conv_band<512, 8><<<c->grid, 512, c->lds_conv, c->stream>>>(a); break;
-      default:
-        if (c->band_wpe == 5)
-          k_conv_band<256, 5><<<c->grid, 256, c->lds_conv, c->stream>>>(a);
-        else if (c->band_wpe == 6)
-          k_conv_band<256, 6><<<c->grid, 256, c->lds_conv, c->stream>>>(a);
-        else if (c->band_wpe == 7)
-          k_conv_band<256, 7><<<c->grid, 256, c->lds_conv, c->stream>>>(a);
-        else
-          k_conv_band<256, 8><<<c->grid, 256, c->lds_conv, c->stream>>>(a);
+      case 64: k_conv_band<64, 8, false><<<c->grid, 64, c->lds_conv, c->stream>>>(a); break;
+      case 128: k_conv_band<128, 8, false><<<c->grid, 128, c->lds_conv, c->stream>>>(a); break;
+      case 512: k_conv_band<512, 8, false><<<c->grid, 512, c->lds_conv, c->stream>>>(a); break;
+      default: {
+        const void *f = conv_kernel_ptr(false, 256, c->band_wpe, c->band_pairs);
+        void *args[] = {const_cast<ConvArgs *>(&a)};
+        HIPCHECK(hipLaunchKernel(f, dim3(c->grid), dim3(256), args, c->lds_conv, c->stream));
         break;
+      }
     }
   }
   HIPCHECK(hipGetLastError());
 }
 
-const void *conv_kernel_ptr(bool staged, int block, int wpe) {
+const void *conv_kernel_ptr(bool staged, int block, int wpe, bool pairs) {
   switch (block) {
-    case 64: return staged ? (const void *)k_conv_compute<64, true> : (const void *)k_conv_band<64, 8>;
-    case 128: return staged ? (const void *)k_conv_compute<128, true> : (const void *)k_conv_band<128, 8>;
-    case 512: return staged ? (const void *)k_conv_compute<512, true> : (const void *)k_conv_band<512, 8>;
+    case 64: return staged ? (const void *)k_conv_compute<64, true> : (const void *)k_conv_band<64, 8, false>;
+    case 128: return staged ? (const void *)k_conv_compute<128, true> : (const void *)k_conv_band<128, 8, false>;
+    case 512: return staged ? (const void *)k_conv_compute<512, true> : (const void *)k_conv_band<512, 8, false>;
     default:
       if (staged) return (const void *)k_conv_compute<256, true>;
-      if (wpe == 5) return (const void *)k_conv_band<256, 5>;
-      if (wpe == 6) return (const void *)k_conv_band<256, 6>;
-      if (wpe == 7) return (const void *)k_conv_band<256, 7>;
-      return (const void *)k_conv_band<256, 8>;
+      if (pairs) {
+        if (wpe == 5) return (const void *)k_conv_band<256, 5, true>;
+        if (wpe == 6) return (const void *)k_conv_band<256, 6, true>;
+        if (wpe == 7) return (const void *)k_conv_band<256, 7, true>;
+        return (const void *)k_conv_band<256, 8, true>;
+      }
+      if (wpe == 5) return (const void *)k_conv_band<256, 5, false>;
+      if (wpe == 6) return (const void *)k_conv_band<256, 6, false>;
+      if (wpe == 7) return (const void *)k_conv_band<256, 7, false>;
+      return (const void *)k_conv_band<256, 8, false>;
   }
 }
 
 void set_lds_attr(fim_ctx *c) {
-  const void *f = conv_kernel_ptr(c->o.staged != 0, c->block, c->band_wpe);
+  const void *f = conv_kernel_ptr(c->o.staged != 0, c->block, c->band_wpe, c->band_pairs);
   HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_conv));
   const void *g = nullptr;
   switch (c->block) {
@@ -1707,7 +1824,12 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         const uint32_t o_geo = o_e2n + (o->staged ? al16(8 * (uint32_t)c->ne_max) : al16(4 * ne32));
         const uint32_t o_list = o_geo + (o->staged ? al16(24 * (uint32_t)c->ne_max) : al16(4 * ne32));
         const uint32_t o_vlist = o_list + al16(2 * (uint32_t)c->ne_max);
-        const uint32_t o_misc = o_vlist + (o->staged ? 0u : al16(2 * nn64));
+        // workgroups of 256 evaluate (element, target) pairs when local
+        // element ids fit 14 bits; the pair rings live in the source list
+        c->band_pairs = !o->staged && c->block == 256 && c->ne_max < (1 << 14);
+        const uint32_t o_misc =
+            o_vlist + (o->staged ? 0u
+                                 : al16(std::max<uint32_t>(2 * nn64, c->band_pairs ? 4u * 2u * kPairRing : 0u)));
         const uint32_t o_wrt = o_misc + 32;  // misc: 8 words
         // unstaged: interface-copy bits and entry counts (2 x nn64/32 words)
         const uint32_t o_ifc = o_wrt + (o->staged ? 0u : al16(4 * (nn64 / 32)));
@@ -1820,7 +1942,8 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         set_lds_attr(c);
         int per_cu = 0;
         HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, conv_kernel_ptr(o->staged != 0, c->block, c->band_wpe), c->block, c->lds_conv));
+            &per_cu, conv_kernel_ptr(o->staged != 0, c->block, c->band_wpe, c->band_pairs), c->block,
+            c->lds_conv));
         per_cu = std::max(per_cu, 1);
         c->grid = o->grid > 0 ? o->grid : ncu * per_cu;
         c->grid = std::max(c->grid, (int)kXcd);  // every XCD eighth needs a workgroup
