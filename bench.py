@@ -190,7 +190,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--max-nn", type=int, default=1408)
+    ap.add_argument("--max-nn", type=int, default=1536)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--local-iters", type=int, default=4,
                     help="sweeps per activation before requeueing (0 = to local convergence)")

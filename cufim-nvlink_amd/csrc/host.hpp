@@ -115,6 +115,10 @@ struct Pack {
 
 Pack build_pack(const Mesh &m, const Partition &p, int32_t source);
 
+// element-list entries the converge-mode band kernel keeps per pass (more
+// marked elements take further passes)
+constexpr int32_t kElistCap = 2048;
+
 // dynamic LDS bytes the converge-mode kernel needs for the largest submesh
 int64_t converge_lds_bytes(int32_t nn_max, int32_t ne_max);
 

@@ -195,7 +195,8 @@ int64_t converge_lds_bytes(int32_t nn_max, int32_t ne_max) {
   const int64_t ne32 = (ne_max + 31) / 32;
   // (the source list doubles as the (element, target) pair rings of the
   // four waves of a workgroup of 256: at least 4 x 320 u16)
-  return 2 * al(4 * nn64) + 4 * al(4 * (nn64 / 32)) + 2 * al(4 * ne32) + al(2LL * ne_max) +
+  return 2 * al(4 * nn64) + 4 * al(4 * (nn64 / 32)) + 2 * al(4 * ne32) +
+         al(2LL * std::min<int64_t>(ne_max, kElistCap)) +
          al(std::max<int64_t>(2 * nn64, ne_max < (1 << 14) ? 2560 : 0)) + 32 + 2 * al(4 * (nn64 / 32));
 }
 

@@ -268,6 +268,7 @@ struct ConvArgs {
   int32_t max_local;
   uint32_t unit, maxch;  // instances per work unit, units per submesh bucket
   uint32_t n_if, wps;    // interface entries; ifm words per submesh
+  uint32_t elcap;        // band kernel: element-list entries per pass
   // LDS carve (bytes)
   uint32_t o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc, o_wrt, o_ifc;
 };
@@ -623,15 +624,6 @@ constexpr uint32_t kAdjPad = 0xffffffffu;
 // left from earlier rounds plus 4 targets of 64 elements
 constexpr uint32_t kPairRing = 320;
 
-// expands the set bits of `bits` (word k of a bitmask) into list[base...]
-__device__ inline void expand_bits(uint32_t bits, uint32_t k, uint16_t *list, uint32_t base) {
-  while (bits) {
-    const uint32_t b = __ffs(bits) - 1;
-    bits &= bits - 1;
-    list[base++] = (uint16_t)(k * 32 + b);
-  }
-}
-
 // marks the (up to two) local elements of one adjacency word in the LDS
 // bitmask; the atomics return nothing, the list is built from the bitmask
 __device__ inline void mark_word(uint32_t *emark, uint32_t w) {
@@ -799,175 +791,192 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
       // elements in id order (clearing the marks, one word per thread)
       for (uint32_t i = tid; i < nsrc; i += BS) mark_incident(a, si.coff + vlist[i], emark);
       __syncthreads();
-      for (uint32_t k = tid; k < new32; k += BS) {
-        const uint32_t bits = emark[k];
-        if (bits) {
-          emark[k] = 0u;
-          eseen[k] |= bits;
-          expand_bits(bits, k, elist, atomicAdd(&misc[0], (uint32_t)__popc(bits)));
-        }
-      }
-      __syncthreads();
-      const uint32_t nact = misc[0];
-      evals += nact;
-      if (tid == 0) misc[2] = 0u;  // every thread has read the source count
-      // candidates of the listed elements
-      if constexpr (PAIRS) {
-        // (element, target) pairs: a pass over 64 listed elements per wave
-        // keeps the targets that pass the bound test and queues them in a
-        // wave-private ring (LDS, in the source list's space, idle until the
-        // accept step); full rounds of 64 pairs then evaluate one target
-        // per lane, so no lane idles on a pruned target
-        uint16_t *pq = vlist + wave * kPairRing;
-        uint32_t qh = 0, qn = 0;  // ring head and fill, uniform across the wave
-        for (uint32_t c0 = wave * 64;; c0 += BS) {
-          const bool more = c0 < nact;  // uniform across the wave
-          if (more) {
-            const uint32_t i = c0 + lane;
-            uint32_t tm = 0, e = 0;
-            if (i < nact) {
-              e = elist[i];
-              const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
-              const uint4 r0 = rec[0], r2 = rec[2];
-              const float hs = csqrt(rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)]);
-              const float h[4] = {__uint_as_float(r2.x) * hs, __uint_as_float(r2.y) * hs,
-                                  __uint_as_float(r2.z) * hs, __uint_as_float(r2.w) * hs};
-              const uint32_t nd[4] = {r0.x & 0x7fffu, (r0.x >> 16) & 0x7fffu, r0.y & 0xffffu,
-                                      r0.y >> 16};
-              float ph[4];
-              int f[4];
-#pragma unroll
-              for (int k = 0; k < 4; ++k) {
-                ph[k] = phi_l[nd[k]];
-                f[k] = flag_at(flg, nd[k]) ? 1 : 0;
+      // element-list passes of at most elcap elements: a word's elements
+      // beyond the cap stay marked for the next pass (the sweep still
+      // evaluates every marked element against the same times)
+      for (;;) {
+        for (uint32_t k = tid; k < new32; k += BS) {
+          uint32_t rest = emark[k];
+          if (rest) {
+            uint32_t pos = atomicAdd(&misc[0], (uint32_t)__popc(rest));
+            if (pos < a.elcap) {
+              uint32_t take = 0;
+              while (rest && pos < a.elcap) {
+                const uint32_t b = __ffs(rest) - 1;
+                rest &= rest - 1;
+                take |= 1u << b;
+                elist[pos++] = (uint16_t)(k * 32 + b);
               }
-              const int nf = f[0] + f[1] + f[2] + f[3];
-              // targets with a source among the other nodes whose bound
-              // (earliest other node + metric height) is still below their time
-              tm |= (nf - f[0] > 0 && ph[0] > fminf(ph[1], fminf(ph[2], ph[3])) + h[0]) ? 1u : 0u;
-              tm |= (nf - f[1] > 0 && ph[1] > fminf(ph[0], fminf(ph[2], ph[3])) + h[1]) ? 2u : 0u;
-              tm |= (nf - f[2] > 0 && ph[2] > fminf(ph[0], fminf(ph[1], ph[3])) + h[2]) ? 4u : 0u;
-              tm |= (nf - f[3] > 0 && ph[3] > fminf(ph[0], fminf(ph[1], ph[2])) + h[3]) ? 8u : 0u;
+              emark[k] = rest;
+              eseen[k] |= take;
             }
-#pragma unroll
-            for (uint32_t T = 0; T < 4; ++T) {
-              const bool p = (tm >> T) & 1u;
-              const uint64_t m = __ballot(p);
-              if (p) {
-                uint32_t pos = qh + qn + (uint32_t)__popcll(m & lanes_below());
+          }
+        }
+        __syncthreads();
+        const uint32_t listed = misc[0];
+        const uint32_t nact = min(listed, a.elcap);
+        evals += nact;
+        if (tid == 0) misc[2] = 0u;  // every thread has read the source count
+        // candidates of the listed elements
+        if constexpr (PAIRS) {
+          // (element, target) pairs: a pass over 64 listed elements per wave
+          // keeps the targets that pass the bound test and queues them in a
+          // wave-private ring (LDS, in the source list's space, idle until the
+          // accept step); full rounds of 64 pairs then evaluate one target
+          // per lane, so no lane idles on a pruned target
+          uint16_t *pq = vlist + wave * kPairRing;
+          uint32_t qh = 0, qn = 0;  // ring head and fill, uniform across the wave
+          for (uint32_t c0 = wave * 64;; c0 += BS) {
+            const bool more = c0 < nact;  // uniform across the wave
+            if (more) {
+              const uint32_t i = c0 + lane;
+              uint32_t tm = 0, e = 0;
+              if (i < nact) {
+                e = elist[i];
+                const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
+                const uint4 r0 = rec[0], r2 = rec[2];
+                const float hs = csqrt(rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)]);
+                const float h[4] = {__uint_as_float(r2.x) * hs, __uint_as_float(r2.y) * hs,
+                                    __uint_as_float(r2.z) * hs, __uint_as_float(r2.w) * hs};
+                const uint32_t nd[4] = {r0.x & 0x7fffu, (r0.x >> 16) & 0x7fffu, r0.y & 0xffffu,
+                                        r0.y >> 16};
+                float ph[4];
+                int f[4];
+  #pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                  ph[k] = phi_l[nd[k]];
+                  f[k] = flag_at(flg, nd[k]) ? 1 : 0;
+                }
+                const int nf = f[0] + f[1] + f[2] + f[3];
+                // targets with a source among the other nodes whose bound
+                // (earliest other node + metric height) is still below their time
+                tm |= (nf - f[0] > 0 && ph[0] > fminf(ph[1], fminf(ph[2], ph[3])) + h[0]) ? 1u : 0u;
+                tm |= (nf - f[1] > 0 && ph[1] > fminf(ph[0], fminf(ph[2], ph[3])) + h[1]) ? 2u : 0u;
+                tm |= (nf - f[2] > 0 && ph[2] > fminf(ph[0], fminf(ph[1], ph[3])) + h[2]) ? 4u : 0u;
+                tm |= (nf - f[3] > 0 && ph[3] > fminf(ph[0], fminf(ph[1], ph[2])) + h[3]) ? 8u : 0u;
+              }
+  #pragma unroll
+              for (uint32_t T = 0; T < 4; ++T) {
+                const bool p = (tm >> T) & 1u;
+                const uint64_t m = __ballot(p);
+                if (p) {
+                  uint32_t pos = qh + qn + (uint32_t)__popcll(m & lanes_below());
+                  if (pos >= kPairRing) pos -= kPairRing;
+                  pq[pos] = (uint16_t)(e | (T << 14));
+                }
+                qn += (uint32_t)__popcll(m);
+              }
+              __builtin_amdgcn_wave_barrier();
+            }
+            while (qn >= 64u || (!more && qn > 0u)) {
+              const uint32_t take = qn < 64u ? qn : 64u;
+              if (lane < take) {
+                uint32_t pos = qh + lane;
                 if (pos >= kPairRing) pos -= kPairRing;
-                pq[pos] = (uint16_t)(e | (T << 14));
-              }
-              qn += (uint32_t)__popcll(m);
-            }
-            __builtin_amdgcn_wave_barrier();
-          }
-          while (qn >= 64u || (!more && qn > 0u)) {
-            const uint32_t take = qn < 64u ? qn : 64u;
-            if (lane < take) {
-              uint32_t pos = qh + lane;
-              if (pos >= kPairRing) pos -= kPairRing;
-              const uint32_t code = pq[pos];
-              const uint32_t e = code & 0x3fffu, T = code >> 14;
-              const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
-              const uint4 r0 = rec[0], r1 = rec[1];
-              const float sc = rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
-              const float g[6] = {__uint_as_float(r0.z) * sc, __uint_as_float(r0.w) * sc,
-                                  __uint_as_float(r1.x) * sc, __uint_as_float(r1.y) * sc,
-                                  __uint_as_float(r1.z) * sc, __uint_as_float(r1.w) * sc};
-              const uint32_t nd[4] = {r0.x & 0x7fffu, (r0.x >> 16) & 0x7fffu, r0.y & 0xffffu,
-                                      r0.y >> 16};
-              // relabel so that the target is node 3 (others T+1, T+2, T+3):
-              // the same candidate as for node position T
-              const bool t1 = T & 1u, t2 = T & 2u;
-              auto sel = [&](float x0, float x1, float x2, float x3) {
-                return t2 ? (t1 ? x3 : x2) : (t1 ? x1 : x0);
-              };
-              auto seli = [&](uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
-                return t2 ? (t1 ? x3 : x2) : (t1 ? x1 : x0);
-              };
-              // edge (i,j) of the relabelled element for T = 0, 1, 2, 3
-              const float gp[6] = {sel(g[2], g[5], g[3], g[0]), sel(g[4], g[1], g[4], g[1]),
-                                   sel(g[5], g[3], g[0], g[2]), sel(g[0], g[2], g[5], g[3]),
-                                   sel(g[1], g[4], g[1], g[4]), sel(g[3], g[0], g[2], g[5])};
-              const uint32_t n1 = seli(nd[1], nd[2], nd[3], nd[0]);
-              const uint32_t n2 = seli(nd[2], nd[3], nd[0], nd[1]);
-              const uint32_t n3 = seli(nd[3], nd[0], nd[1], nd[2]);
-              const uint32_t nt = seli(nd[0], nd[1], nd[2], nd[3]);
-              const float pp[4] = {phi_l[n1], phi_l[n2], phi_l[n3], phi_l[nt]};
-              float sq[6], rc[6];
-#pragma unroll
-              for (int k = 0; k < 3; ++k) {
-                rc[k] = __builtin_amdgcn_rcpf(gp[k]);
-                sq[k + 3] = csqrt(gp[k + 3]);
-                sq[k] = 0.0f;
-                rc[k + 3] = 0.0f;
-              }
-              const float c = cand_exact_sh<3, float>(gp, sq, rc, pp, F_INF);
-              ++vupd;
-              if (c < pp[3]) {
-                const uint32_t cbits = __float_as_uint(c);
-                if (cbits < atomicMin(&phi_n[nt], cbits)) atomicOr(&wrt[nt >> 5], 1u << (nt & 31));
-              }
-            }
-            qh += take;
-            if (qh >= kPairRing) qh -= kPairRing;
-            qn -= take;
-            __builtin_amdgcn_wave_barrier();
-          }
-          if (!more) break;
-        }
-      } else {
-        for (uint32_t i = tid; i < nact; i += BS) {
-          const uint32_t e = elist[i];
-          const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
-          const uint4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-          const float sc = rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
-          const float g[6] = {__uint_as_float(r0.z) * sc, __uint_as_float(r0.w) * sc,
-                              __uint_as_float(r1.x) * sc, __uint_as_float(r1.y) * sc,
-                              __uint_as_float(r1.z) * sc, __uint_as_float(r1.w) * sc};
-          // square roots (edge candidates) and reciprocals (triangle bases) of
-          // the six squared lengths, shared by the element's four targets
-          float sq[6], rc[6];
+                const uint32_t code = pq[pos];
+                const uint32_t e = code & 0x3fffu, T = code >> 14;
+                const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
+                const uint4 r0 = rec[0], r1 = rec[1];
+                const float sc = rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
+                const float g[6] = {__uint_as_float(r0.z) * sc, __uint_as_float(r0.w) * sc,
+                                    __uint_as_float(r1.x) * sc, __uint_as_float(r1.y) * sc,
+                                    __uint_as_float(r1.z) * sc, __uint_as_float(r1.w) * sc};
+                const uint32_t nd[4] = {r0.x & 0x7fffu, (r0.x >> 16) & 0x7fffu, r0.y & 0xffffu,
+                                        r0.y >> 16};
+                // relabel so that the target is node 3 (others T+1, T+2, T+3):
+                // the same candidate as for node position T
+                const bool t1 = T & 1u, t2 = T & 2u;
+                auto sel = [&](float x0, float x1, float x2, float x3) {
+                  return t2 ? (t1 ? x3 : x2) : (t1 ? x1 : x0);
+                };
+                auto seli = [&](uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
+                  return t2 ? (t1 ? x3 : x2) : (t1 ? x1 : x0);
+                };
+                // edge (i,j) of the relabelled element for T = 0, 1, 2, 3
+                const float gp[6] = {sel(g[2], g[5], g[3], g[0]), sel(g[4], g[1], g[4], g[1]),
+                                     sel(g[5], g[3], g[0], g[2]), sel(g[0], g[2], g[5], g[3]),
+                                     sel(g[1], g[4], g[1], g[4]), sel(g[3], g[0], g[2], g[5])};
+                const uint32_t n1 = seli(nd[1], nd[2], nd[3], nd[0]);
+                const uint32_t n2 = seli(nd[2], nd[3], nd[0], nd[1]);
+                const uint32_t n3 = seli(nd[3], nd[0], nd[1], nd[2]);
+                const uint32_t nt = seli(nd[0], nd[1], nd[2], nd[3]);
+                const float pp[4] = {phi_l[n1], phi_l[n2], phi_l[n3], phi_l[nt]};
+                float sq[6], rc[6];
   #pragma unroll
-          for (int k = 0; k < 6; ++k) {
-            sq[k] = csqrt(g[k]);
-            rc[k] = __builtin_amdgcn_rcpf(g[k]);
+                for (int k = 0; k < 3; ++k) {
+                  rc[k] = __builtin_amdgcn_rcpf(gp[k]);
+                  sq[k + 3] = csqrt(gp[k + 3]);
+                  sq[k] = 0.0f;
+                  rc[k + 3] = 0.0f;
+                }
+                const float c = cand_exact_sh<3, float>(gp, sq, rc, pp, F_INF);
+                ++vupd;
+                if (c < pp[3]) {
+                  const uint32_t cbits = __float_as_uint(c);
+                  if (cbits < atomicMin(&phi_n[nt], cbits)) atomicOr(&wrt[nt >> 5], 1u << (nt & 31));
+                }
+              }
+              qh += take;
+              if (qh >= kPairRing) qh -= kPairRing;
+              qn -= take;
+              __builtin_amdgcn_wave_barrier();
+            }
+            if (!more) break;
           }
-          const float hs = csqrt(sc);
-          const float h[4] = {__uint_as_float(r2.x) * hs, __uint_as_float(r2.y) * hs,
-                              __uint_as_float(r2.z) * hs, __uint_as_float(r2.w) * hs};
-          const uint32_t nd[4] = {r0.x & 0x7fffu, (r0.x >> 16) & 0x7fffu, r0.y & 0xffffu,
-                                  r0.y >> 16};
-          float ph[4];
-          int f[4];
-  #pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            ph[k] = phi_l[nd[k]];
-            f[k] = flag_at(flg, nd[k]) ? 1 : 0;
+        } else {
+          for (uint32_t i = tid; i < nact; i += BS) {
+            const uint32_t e = elist[i];
+            const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
+            const uint4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+            const float sc = rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
+            const float g[6] = {__uint_as_float(r0.z) * sc, __uint_as_float(r0.w) * sc,
+                                __uint_as_float(r1.x) * sc, __uint_as_float(r1.y) * sc,
+                                __uint_as_float(r1.z) * sc, __uint_as_float(r1.w) * sc};
+            // square roots (edge candidates) and reciprocals (triangle bases) of
+            // the six squared lengths, shared by the element's four targets
+            float sq[6], rc[6];
+    #pragma unroll
+            for (int k = 0; k < 6; ++k) {
+              sq[k] = csqrt(g[k]);
+              rc[k] = __builtin_amdgcn_rcpf(g[k]);
+            }
+            const float hs = csqrt(sc);
+            const float h[4] = {__uint_as_float(r2.x) * hs, __uint_as_float(r2.y) * hs,
+                                __uint_as_float(r2.z) * hs, __uint_as_float(r2.w) * hs};
+            const uint32_t nd[4] = {r0.x & 0x7fffu, (r0.x >> 16) & 0x7fffu, r0.y & 0xffffu,
+                                    r0.y >> 16};
+            float ph[4];
+            int f[4];
+    #pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              ph[k] = phi_l[nd[k]];
+              f[k] = flag_at(flg, nd[k]) ? 1 : 0;
+            }
+            const int nf = f[0] + f[1] + f[2] + f[3];
+            // targets with a source among the other nodes whose bound (earliest
+            // other node + metric height) is still below their time
+    #define FIM_TARGET(T, A, B, C)                                                        \
+            if (nf - f[T] > 0 && ph[T] > fminf(ph[A], fminf(ph[B], ph[C])) + h[T]) {         \
+              const float c = cand_exact_sh<T, float>(g, sq, rc, ph, F_INF);                 \
+              ++vupd;                                                                         \
+              if (c < ph[T]) {                                                                \
+                const uint32_t cbits = __float_as_uint(c);                                    \
+                if (cbits < atomicMin(&phi_n[nd[T]], cbits))                                  \
+                  atomicOr(&wrt[nd[T] >> 5], 1u << (nd[T] & 31));                             \
+              }                                                                               \
+            }
+            FIM_TARGET(0, 1, 2, 3)
+            FIM_TARGET(1, 0, 2, 3)
+            FIM_TARGET(2, 0, 1, 3)
+            FIM_TARGET(3, 0, 1, 2)
+    #undef FIM_TARGET
           }
-          const int nf = f[0] + f[1] + f[2] + f[3];
-          // targets with a source among the other nodes whose bound (earliest
-          // other node + metric height) is still below their time
-  #define FIM_TARGET(T, A, B, C)                                                        \
-          if (nf - f[T] > 0 && ph[T] > fminf(ph[A], fminf(ph[B], ph[C])) + h[T]) {         \
-            const float c = cand_exact_sh<T, float>(g, sq, rc, ph, F_INF);                 \
-            ++vupd;                                                                         \
-            if (c < ph[T]) {                                                                \
-              const uint32_t cbits = __float_as_uint(c);                                    \
-              if (cbits < atomicMin(&phi_n[nd[T]], cbits))                                  \
-                atomicOr(&wrt[nd[T] >> 5], 1u << (nd[T] & 31));                             \
-            }                                                                               \
-          }
-          FIM_TARGET(0, 1, 2, 3)
-          FIM_TARGET(1, 0, 2, 3)
-          FIM_TARGET(2, 0, 1, 3)
-          FIM_TARGET(3, 0, 1, 2)
-  #undef FIM_TARGET
         }
+        __syncthreads();
+        if (tid == 0) misc[0] = 0u;  // every thread has read the element count
+        if (listed <= a.elcap) break;  // uniform: read after a barrier
+        __syncthreads();
       }
-      __syncthreads();
-      if (tid == 0) misc[0] = 0u;  // every thread has read the element count
       // accept the written vertices: the sources of the next sweep
       for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
         const uint32_t l = l0 + tid;
@@ -1823,7 +1832,10 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         // staged: element nodes + geometry; unstaged: element mark bitmasks
         const uint32_t o_geo = o_e2n + (o->staged ? al16(8 * (uint32_t)c->ne_max) : al16(4 * ne32));
         const uint32_t o_list = o_geo + (o->staged ? al16(24 * (uint32_t)c->ne_max) : al16(4 * ne32));
-        const uint32_t o_vlist = o_list + al16(2 * (uint32_t)c->ne_max);
+        // the band kernel lists at most kElistCap elements per pass
+        const uint32_t elcap = o->staged ? (uint32_t)c->ne_max
+                                         : std::min<uint32_t>((uint32_t)c->ne_max, (uint32_t)kElistCap);
+        const uint32_t o_vlist = o_list + al16(2 * elcap);
         // workgroups of 256 evaluate (element, target) pairs when local
         // element ids fit 14 bits; the pair rings live in the source list
         c->band_pairs = !o->staged && c->block == 256 && c->ne_max < (1 << 14);
@@ -1954,7 +1966,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
                          c->d_n2e_dsp, c->d_n2e_le, c->d_erec, c->d_adj_ptr, c->d_adj,
                          c->nsub, c->ncopy, c->ninst, (float)(o->eps_rel > 0 ? o->eps_rel : 1e-6),
                          (float)o->window, o->local_iters, c->unit, c->maxch, (uint32_t)p.if_l.size(),
-                         c->wps, o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc,
+                         c->wps, elcap, o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc,
                          o_wrt, o_ifc};
         c->sa = SyncArgs{c->d_sub, c->d_if_dsp, c->d_if_os, c->d_prec, c->d_pcnt,
                          c->d_phi, c->d_dirty, c->d_act, c->d_wake, WorkList{}, WorkList{},

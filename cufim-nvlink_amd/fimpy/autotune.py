@@ -4,7 +4,7 @@ the compute kernel's register budget for a mesh and batch size by measuring
 short batched solves on the GPU.
 
 Search: coordinate descent from the defaults over
-    max_nn      {256, 512, 1024, 1408, 2048}  vertex copies per submesh
+    max_nn      {256, 512, 1024, 1536, 2048}  vertex copies per submesh
     block       {64, 128, 256}                workgroup size
     local_iters {0, 1, 2, 4, 8}               sweeps per activation (0 = to local convergence)
     window      {500, 1000, 2000, 4000}       microseconds per global iteration
@@ -25,13 +25,13 @@ import numpy as np
 from . import PART_RCB, Mesh, Pack, Solver
 
 SPACE = {
-    "max_nn": [256, 512, 1024, 1408, 2048],
+    "max_nn": [256, 512, 1024, 1536, 2048],
     "block": [64, 128, 256],
     "local_iters": [0, 1, 2, 4, 8],
     "window": [500.0, 1000.0, 2000.0, 4000.0],
     "waves": [0, 5],
 }
-DEFAULT = {"max_nn": 1408, "block": 256, "local_iters": 4, "window": 2000.0, "waves": 0}
+DEFAULT = {"max_nn": 1536, "block": 256, "local_iters": 4, "window": 2000.0, "waves": 0}
 
 
 def signature(mesh: Mesh, n_instances: int) -> str:
