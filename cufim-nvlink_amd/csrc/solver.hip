@@ -982,8 +982,16 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
         const uint32_t l = l0 + tid;
         const uint32_t wb = l0 + wave * 64;
         if (wb < nv64) {
-          bool sg = false, lt = false, dc = false;
           const uint32_t k = wb >> 5;
+          // a block of 64 copies with nothing written has no sources
+          if ((wrt[k] | wrt[k + 1]) == 0u) {  // uniform: same LDS words
+            if (lane == 0) {
+              flg[k] = 0u;
+              flg[k + 1] = 0u;
+            }
+            continue;
+          }
+          bool sg = false, lt = false, dc = false;
           if (l < nn && ((wrt[l >> 5] >> (l & 31)) & 1u)) {
             const float ov = phi_l[l];
             const float nv = __uint_as_float(phi_n[l]);
@@ -1030,6 +1038,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
     uint2 *rec = a.prec + (size_t)inst * a.n_if + si.ioff;
     for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
       const uint32_t l = l0 + tid;  // uniform trip count: ballots below
+      const uint32_t k = (l0 + wave * 64) >> 5;
+      // blocks of 64 copies with nothing lowered or left dirty are skipped
+      if (l0 + wave * 64 >= nv64 ||
+          (tch[k] | tch[k + 1] | dfr[k] | dfr[k + 1] | (capped ? (flg[k] | flg[k + 1]) : 0u)) == 0u)
+        continue;  // uniform across the wave
       bool p = false;
       if (l < nn) {
         if (flag_at(tch, l)) {
