@@ -663,7 +663,7 @@ __device__ inline void wave_append(bool pred, uint32_t l, uint16_t *list, uint32
   const uint64_t m = __ballot(pred);
   uint32_t base = 0;
   if ((threadIdx.x & 63) == 0 && m) base = atomicAdd(cnt, (uint32_t)__popcll(m));
-  base = __shfl(base, 0);
+  base = __builtin_amdgcn_readfirstlane(base);  // every lane of the wave is active
   if (pred) list[base + __popcll(m & lanes_below())] = (uint16_t)l;
 }
 
@@ -796,20 +796,30 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
       // evaluates every marked element against the same times)
       for (;;) {
         for (uint32_t k = tid; k < new32; k += BS) {
+          // list offsets: a wave prefix sum of the word counts (<= 32, six
+          // bits) from one ballot per bit, one LDS reservation per wave
           uint32_t rest = emark[k];
-          if (rest) {
-            uint32_t pos = atomicAdd(&misc[0], (uint32_t)__popc(rest));
-            if (pos < a.elcap) {
-              uint32_t take = 0;
-              while (rest && pos < a.elcap) {
-                const uint32_t b = __ffs(rest) - 1;
-                rest &= rest - 1;
-                take |= 1u << b;
-                elist[pos++] = (uint16_t)(k * 32 + b);
-              }
-              emark[k] = rest;
-              eseen[k] |= take;
+          const uint32_t n = __popc(rest);
+          uint32_t pre = 0, tot = 0;
+#pragma unroll
+          for (int b = 0; b < 6; ++b) {
+            const uint64_t m = __ballot((n >> b) & 1u);
+            pre += (uint32_t)__popcll(m & lanes_below()) << b;
+            tot += (uint32_t)__popcll(m) << b;
+          }
+          uint32_t base = 0;
+          if (lane == 0 && tot) base = atomicAdd(&misc[0], tot);  // lane 0 is active here
+          uint32_t pos = __builtin_amdgcn_readfirstlane(base) + pre;
+          if (rest && pos < a.elcap) {
+            uint32_t take = 0;
+            while (rest && pos < a.elcap) {
+              const uint32_t b = __ffs(rest) - 1;
+              rest &= rest - 1;
+              take |= 1u << b;
+              elist[pos++] = (uint16_t)(k * 32 + b);
             }
+            emark[k] = rest;
+            eseen[k] |= take;
           }
         }
         __syncthreads();
@@ -1058,7 +1068,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
       if (m) {
         uint32_t wb = 0;
         if (lane == 0) wb = atomicAdd(&misc[6], (uint32_t)__popcll(m));
-        wb = __shfl(wb, 0);
+        wb = __builtin_amdgcn_readfirstlane(wb);  // every lane of the wave is active
         if (p) {
           const uint32_t w = l >> 5;
           const uint32_t j = ipre[w] + __popc(ifc[w] & ((1u << (l & 31)) - 1u));
