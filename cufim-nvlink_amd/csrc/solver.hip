@@ -657,6 +657,16 @@ __device__ inline void mark_incident(const ConvArgs &a, uint32_t c, uint32_t *em
   for (uint32_t k = 4; k < nch; ++k) mark_chunk(emark, ch[k]);
 }
 
+// sum over the wave of n < 2^BITS, one ballot per bit; every lane of the
+// wave must be active
+template <int BITS>
+__device__ inline uint32_t wave_sum_small(uint32_t n) {
+  uint32_t s = 0;
+#pragma unroll
+  for (int b = 0; b < BITS; ++b) s += (uint32_t)__popcll(__ballot((n >> b) & 1u)) << b;
+  return s;
+}
+
 // appends the lanes with `pred` (vertex l) to list[*cnt ...], one LDS atomic
 // per wave
 __device__ inline void wave_append(bool pred, uint32_t l, uint16_t *list, uint32_t *cnt) {
@@ -1040,12 +1050,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
     // (4+4+4) and the distinct elements evaluated (48 B record)
     uint32_t cnt = 0;
     for (uint32_t k = tid; k < new32; k += BS) cnt += __popc(eseen[k]);
-    if (cnt) atomicAdd(&misc[3], cnt);
+    cnt = wave_sum_small<9>(cnt);  // at most 8 words of 32 per thread
+    if (lane == 0 && cnt) atomicAdd(&misc[3], cnt);
     // write back decreased copies, lowered interface copies as push records
     // into the item's slot range (entry = interface copies before it);
     // deferred vertices (and sources left by the sweep cap) stay dirty and
     // the submesh is queued again
     uint2 *rec = a.prec + (size_t)inst * a.n_if + si.ioff;
+    uint32_t dmin = 0xffffffffu;  // earliest time left dirty, per lane
     for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
       const uint32_t l = l0 + tid;  // uniform trip count: ballots below
       const uint32_t k = (l0 + wave * 64) >> 5;
@@ -1061,7 +1073,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
         }
         if (flag_at(dfr, l) || (capped && flag_at(flg, l))) {
           a.dirty[cb + l] = 1;
-          atomicMin(&misc[5], __float_as_uint(phi_l[l]));
+          dmin = min(dmin, __float_as_uint(phi_l[l]));
         }
       }
       const uint64_t m = __ballot(p);
@@ -1076,6 +1088,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
         }
       }
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) dmin = min(dmin, (uint32_t)__shfl_xor(dmin, off));
+    if (lane == 0 && dmin != 0xffffffffu) atomicMin(&misc[5], dmin);
     __syncthreads();
     if (tid == 0) {
       a.pcnt[entry] = misc[6];
