@@ -137,6 +137,31 @@ def test_converge_workgroup_shapes_agree(block, staged):
     _check_fp32(phi, ref)
 
 
+def test_converge_long_element_lists_and_pair_path_agree():
+    # 2048-copy submeshes seeded on their whole bottom faces: sweeps mark more
+    # elements than one element-list pass holds, so they run in several
+    # passes.  Workgroups of 256 evaluate (element, target) pairs, of 128 one
+    # element per lane: the same sweeps, so bit-identical maps, and both
+    # within tolerance of the FP64 oracle.
+    cfg = cases.scaled(cases.SLAB, 4)
+    m, om = cases.box(cfg, seed=17)
+    p = F.Pack(m, F.PART_RCB, max_nn=2048, seed=17)
+    assert p.info["ne_max"] < (1 << 14)  # the pair path is taken at 256
+    xyz, _, _ = m.arrays()
+    src = np.nonzero(xyz[:, 2] < xyz[:, 2].min() + 0.2 * cfg["lz"] / cfg["nz"])[0]
+    assert len(src) > 1000  # a bottom layer of ~2,400 tets per submesh is marked at once
+    maps = {}
+    for block in (128, 256):
+        s = F.Solver(p, block=block, window=0.0, local_iters=4)
+        s.seed(src, 0.0)
+        s.run()
+        maps[block] = s.download()[0]
+        s.close()
+    assert np.array_equal(maps[128], maps[256])
+    ref, _ = O.fim_solve(om, src, variant=O.LOCAL_EXACT)
+    _check_fp32(maps[256], ref)
+
+
 @pytest.mark.parametrize("max_nn", [256, 1280])
 def test_converge_register_variants_agree(max_nn):
     # the compute kernel built for 5..8 waves per SIMD: same arithmetic, so the
