@@ -201,6 +201,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--graph-steps", type=int, default=0,
                     help="also time this many steps with graph-captured iterations")
+    ap.add_argument("--stream-steps", type=int, default=0,
+                    help="also time this many batches of maps streamed through the instances "
+                         "(continuous batching: a converged instance takes the next map)")
     ap.add_argument("--staged", action="store_true",
                     help="stage whole submeshes in LDS (default: element data from L2)")
     ap.add_argument("--window", type=float, default=2000.0,
@@ -277,6 +280,21 @@ def main():
     s.close()
     ms_per_step = 1e3 * elapsed / args.steps
     maps_per_s = args.batch * args.steps / elapsed
+
+    # ---- optionally: continuous batching of stream_steps x batch further
+    # maps through the same number of instances (graph-captured iterations)
+    stream_maps_per_s = None
+    if args.stream_steps > 0:
+        from fimpy.stream import stream_solve
+        g = F.Solver(pack, n_instances=args.batch, block=args.block, window=args.window, unit=args.unit,
+                     local_iters=args.local_iters, staged=args.staged)
+        ssets = pacing_sets(cfg, args.batch * (args.stream_steps + 1), args.seed + 1)
+        stream_solve(g, ssets[:args.batch])  # warm-up
+        t0 = time.perf_counter()
+        stream_solve(g, ssets[args.batch:])
+        stream_maps_per_s = args.batch * args.stream_steps / (time.perf_counter() - t0)
+        g.close()
+        log(f"[bench] streamed {args.batch * args.stream_steps} maps: {stream_maps_per_s:.1f} maps/s")
 
     # ---- optionally: the same batch with graph-captured iterations (the
     # library default; its launches cannot carry events, so it is off here
@@ -388,6 +406,7 @@ def main():
         "valid": valid,
         "residual_inst0": res0,
         "graph_maps_per_s": graph_maps_per_s,
+        "stream_maps_per_s": stream_maps_per_s,
         "plain_maps_per_s": 1.0 / plain_s,
         "plain_config": {"max_nn": args.plain_max_nn, "window_us": args.plain_window,
                          "local_iters": args.plain_local_iters, "waves": args.plain_waves},

@@ -257,6 +257,8 @@ struct ConvArgs {
   unsigned long long *stats;  // activations, sweeps, element evaluations,
                               // iterations, algorithmic bytes
   const uint32_t *iter;  // global iteration counter (advanced by the sync kernel)
+  const uint32_t *start; // n_inst: iteration an instance was (re)seeded at; its
+                         // window is window * (iter - start + 1)
   const uint32_t *n2e_dsp;  // per copy: incident local elements (staged kernel)
   const uint16_t *n2e_le;
   const uint4 *erec;        // band kernel: 48 B per element (ElemRecord layout)
@@ -331,11 +333,12 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
   if (blockIdx.x == 0 && tid == 0 && cursor.U > 0) atomicAdd(&a.stats[3], 1ull);
   unsigned long long sweeps = 0, evals = 0, acts = 0, abytes = 0, vupd = 0;
   // time window of this iteration: flagged vertices later than W wait
-  const float W = a.window > 0.0f ? a.window * (float)(*a.iter + 1u) : F_INF;
+  const uint32_t iter_now = *a.iter;
 
   uint32_t inst, s;
   while (cursor.next(inst, s)) {
     const size_t entry = (size_t)inst * a.nsub + s;
+    const float W = a.window > 0.0f ? a.window * (float)(iter_now - a.start[inst] + 1u) : F_INF;
     const SubInfo si = a.sub[s];
     const size_t cb = (size_t)inst * a.ncopy + si.coff;
     const uint32_t nn = si.nn, ne = si.ne;
@@ -717,11 +720,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
   UnitCursor cursor(a.cur, &misc[4], a.ninst, a.unit, a.maxch);
   if (blockIdx.x == 0 && tid == 0 && cursor.U > 0) atomicAdd(&a.stats[3], 1ull);
   unsigned long long sweeps = 0, evals = 0, acts = 0, abytes = 0, vupd = 0;
-  const float W = a.window > 0.0f ? a.window * (float)(*a.iter + 1u) : F_INF;
+  const uint32_t iter_now = *a.iter;
 
   uint32_t inst, s;
   while (cursor.next(inst, s)) {
     const size_t entry = (size_t)inst * a.nsub + s;
+    const float W = a.window > 0.0f ? a.window * (float)(iter_now - a.start[inst] + 1u) : F_INF;
     const SubInfo si = a.sub[s];
     const size_t cb = (size_t)inst * a.ncopy + si.coff;
     const uint32_t nn = si.nn, ne = si.ne;
@@ -1189,6 +1193,38 @@ __global__ void k_conv_seed(const uint2 *ic, uint32_t n, float t0, const uint32_
   }
 }
 
+// per instance: 1 if any (instance, submesh) pair is queued
+__global__ __launch_bounds__(256) void k_inst_active(const uint32_t *act, uint32_t nsub, uint32_t *out) {
+  __shared__ uint32_t any;
+  if (threadIdx.x == 0) any = 0u;
+  __syncthreads();
+  const uint32_t *row = act + (size_t)blockIdx.x * nsub;
+  uint32_t v = 0u;
+  for (uint32_t s = threadIdx.x; s < nsub; s += blockDim.x) v |= row[s];
+  if (v) atomicOr(&any, 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = any;
+}
+
+// recycles finished instances: times to +inf, flags and wake times cleared,
+// and the instance's window restarts at the current iteration
+__global__ void k_inst_reset(const uint32_t *insts, uint32_t n, float *phi, uint8_t *dirty, float *wake,
+                             uint32_t *start, const uint32_t *iter, uint32_t ncopy, uint32_t nsub) {
+  const size_t per = (size_t)ncopy + nsub;
+  const size_t total = per * n;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t inst = insts[i / per];
+    const size_t k = i % per;
+    if (k < ncopy) {
+      phi[(size_t)inst * ncopy + k] = F_INF;
+      dirty[(size_t)inst * ncopy + k] = 0;
+    } else {
+      wake[(size_t)inst * nsub + (k - ncopy)] = F_INF;
+    }
+    if (k == 0) start[inst] = *iter;
+  }
+}
+
 // warm start: every copy of the instance takes the global field value,
 // everything flagged, every submesh queued
 __global__ void k_conv_init(const float *phi0, const uint32_t *c2g, uint32_t inst, float *phi,
@@ -1577,6 +1613,8 @@ struct fim_ctx {
   float *d_phi = nullptr, *d_rscale = nullptr;
   uint8_t *d_dirty = nullptr;
   uint32_t *d_act = nullptr, *d_cnt = nullptr;
+  uint32_t *d_start = nullptr;  // n_inst: iteration each instance was (re)seeded at
+  uint32_t *d_flag = nullptr;   // n_inst: scratch for instance queries / lists
   uint2 *d_prec = nullptr;  // push records: n_inst x (interface entries)
   uint32_t *d_pcnt = nullptr;
   uint2 *d_ifm = nullptr;
@@ -1942,6 +1980,9 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
           // push records: one slot per (instance, interface entry)
           c->d_prec = dalloc<uint2>((size_t)c->ninst * p.if_l.size(), ow);
           c->d_pcnt = dalloc<uint32_t>(nss, ow);
+          c->d_start = dalloc<uint32_t>(c->ninst, ow);
+          c->d_flag = dalloc<uint32_t>(c->ninst, ow);
+          HIPCHECK(hipMemset(c->d_start, 0, sizeof(uint32_t) * c->ninst));
           // per submesh: interface-copy bits of each 32 local copies and the
           // number of interface entries before that word (if_l ascends)
           c->wps = (uint32_t)((c->nn_max + 63) / 64 * 2);
@@ -2000,7 +2041,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         c->lds_compat = 0;
         c->ca = ConvArgs{c->d_sub, c->d_e2n, c->d_geo32, c->d_hgt, c->d_ereg, c->d_rscale, c->d_phi,
                          c->d_dirty, c->d_act, c->d_wake, c->d_if_l, c->d_ifm, c->d_prec,
-                         c->d_pcnt, WorkList{}, WorkList{}, c->d_stats, c->d_cnt + 48,
+                         c->d_pcnt, WorkList{}, WorkList{}, c->d_stats, c->d_cnt + 48, c->d_start,
                          c->d_n2e_dsp, c->d_n2e_le, c->d_erec, c->d_adj_ptr, c->d_adj,
                          c->nsub, c->ncopy, c->ninst, (float)(o->eps_rel > 0 ? o->eps_rel : 1e-6),
                          (float)o->window, o->local_iters, c->unit, c->maxch, (uint32_t)p.if_l.size(),
@@ -2067,6 +2108,7 @@ int fim_reset(fim_ctx *c) {
       k_fill<uint32_t><<<blocks_for(nss, 256), 256, 0, c->stream>>>(c->d_act, nss, 0u);
       k_fill<float><<<blocks_for(nss, 256), 256, 0, c->stream>>>(c->d_wake, nss, F_INF);
       k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_cnt, 64, 0u);
+      k_fill<uint32_t><<<blocks_for(c->ninst, 256), 256, 0, c->stream>>>(c->d_start, c->ninst, 0u);
       for (int k = 0; k < 3; ++k)
         k_fill<uint32_t><<<blocks_for(c->nsub, 256), 256, 0, c->stream>>>(c->wl[k].bcnt, c->nsub, 0u);
     } else {
@@ -2205,64 +2247,141 @@ int fim_step(fim_ctx *c, double phi_start, double cur_time, double dt, double fr
   });
 }
 
+namespace {
+// Runs global iterations in graph-sized chunks until no work is queued or
+// max_iters (rounded up to whole chunks) have run; returns the queued units
+// left (0 = every instance converged).  Device counters accumulate from the
+// last reset_stats call.
+uint32_t run_iterations(fim_ctx *c, int32_t max_iters, bool reset_stats) {
+  HIPCHECK(hipSetDevice(c->dev));
+  // profile mode launches the same sequence of kernels without the graph
+  // (events cannot be timed inside a graph) and times every compute launch
+  if (c->o.profile && c->kev.empty()) {
+    c->kev.assign(2 * c->graph_iters, nullptr);
+    for (auto &e : c->kev) HIPCHECK(hipEventCreate(&e));
+  }
+  if (!c->o.profile && !c->gexec) build_graph(c);
+  if (reset_stats) {
+    HIPCHECK(hipMemsetAsync(c->d_stats, 0, 8 * sizeof(unsigned long long), c->stream));
+    c->stats.compute_ms = 0.0;
+    c->stats.compute_launches = 0;
+  }
+  HIPCHECK(hipEventRecord(c->ev0, c->stream));
+  // units of the list the seeds (or an earlier run) filled
+  launch_plan(c, 0, -1);
+  int32_t done = 0;
+  uint32_t left = 1;
+  double kms = 0.0;
+  int64_t klaunch = 0;
+  while (done < max_iters) {
+    if (c->o.profile) {
+      for (int i = 0; i < c->graph_iters; ++i) enqueue_iteration(c, i, true);
+    } else {
+      HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+    }
+    done += c->graph_iters;
+    HIPCHECK(hipMemcpyAsync(c->h_pin, c->d_cnt, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (c->o.profile) {
+      for (int j = 0; j < c->graph_iters; ++j) {
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, c->kev[2 * j], c->kev[2 * j + 1]));
+        kms += ms;
+        ++klaunch;
+      }
+    }
+    left = c->h_pin[0];
+    if (left == 0) break;
+  }
+  HIPCHECK(hipEventRecord(c->ev1, c->stream));
+  unsigned long long st[8];
+  HIPCHECK(hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->stats.last_ms = ms;
+  c->stats.activations = (int64_t)st[0];
+  c->stats.local_iters = (int64_t)st[1];
+  c->stats.elem_updates = (int64_t)st[2];
+  c->stats.global_iters = (int64_t)st[3];
+  c->stats.alg_bytes = (int64_t)st[4];
+  c->stats.vertex_updates = (int64_t)st[5];
+  c->stats.compute_ms += kms;
+  c->stats.compute_launches += klaunch;
+  return left;
+}
+
+std::vector<uint32_t> checked_instances(const fim_ctx *c, int32_t n, const int32_t *insts) {
+  if (n < 0 || (n > 0 && !insts)) throw Error(FIM_EINVAL, "bad instance list");
+  std::vector<uint32_t> v(n);
+  for (int32_t i = 0; i < n; ++i) {
+    check_inst(c, insts[i]);
+    v[i] = (uint32_t)insts[i];
+  }
+  return v;
+}
+}  // namespace
+
 int fim_run(fim_ctx *c, int32_t max_iters, int32_t *iters) {
   return guarded([&] {
     if (!c) throw Error(FIM_EINVAL, "null context");
     if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "fim_run needs converge mode");
     if (max_iters <= 0) throw Error(FIM_EINVAL, "max_iters must be positive");
-    HIPCHECK(hipSetDevice(c->dev));
-    // profile mode launches the same sequence of kernels without the graph
-    // (events cannot be timed inside a graph) and times every compute launch
-    if (c->o.profile && c->kev.empty()) {
-      c->kev.assign(2 * c->graph_iters, nullptr);
-      for (auto &e : c->kev) HIPCHECK(hipEventCreate(&e));
-    }
-    if (!c->o.profile && !c->gexec) build_graph(c);
-    HIPCHECK(hipMemsetAsync(c->d_stats, 0, 8 * sizeof(unsigned long long), c->stream));
-    HIPCHECK(hipEventRecord(c->ev0, c->stream));
-    // units of the list the seeds (or an earlier capped run) filled
-    launch_plan(c, 0, -1);
-    int32_t done = 0;
-    uint32_t left = 1;
-    double kms = 0.0;
-    int64_t klaunch = 0;
-    while (done < max_iters) {
-      if (c->o.profile) {
-        for (int i = 0; i < c->graph_iters; ++i) enqueue_iteration(c, i, true);
-      } else {
-        HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
-      }
-      done += c->graph_iters;
-      HIPCHECK(hipMemcpyAsync(c->h_pin, c->d_cnt, 4, hipMemcpyDeviceToHost, c->stream));
-      HIPCHECK(hipStreamSynchronize(c->stream));
-      if (c->o.profile) {
-        for (int j = 0; j < c->graph_iters; ++j) {
-          float ms = 0;
-          HIPCHECK(hipEventElapsedTime(&ms, c->kev[2 * j], c->kev[2 * j + 1]));
-          kms += ms;
-          ++klaunch;
-        }
-      }
-      left = c->h_pin[0];
-      if (left == 0) break;
-    }
-    HIPCHECK(hipEventRecord(c->ev1, c->stream));
-    unsigned long long st[8];
-    HIPCHECK(hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
-    float ms = 0;
-    HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->stats.last_ms = ms;
-    c->stats.activations = (int64_t)st[0];
-    c->stats.local_iters = (int64_t)st[1];
-    c->stats.elem_updates = (int64_t)st[2];
-    c->stats.global_iters = (int64_t)st[3];
-    c->stats.alg_bytes = (int64_t)st[4];
-    c->stats.vertex_updates = (int64_t)st[5];
-    c->stats.compute_ms = kms;
-    c->stats.compute_launches = klaunch;
-    if (iters) *iters = (int32_t)st[3];
+    const uint32_t left = run_iterations(c, max_iters, true);
+    if (iters) *iters = (int32_t)c->stats.global_iters;
     if (left != 0) throw Error(FIM_ENOCONV, "iteration cap reached before convergence");
+  });
+}
+
+int fim_advance(fim_ctx *c, int32_t iters, int32_t reset_stats, int32_t *left) {
+  return guarded([&] {
+    if (!c) throw Error(FIM_EINVAL, "null context");
+    if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "fim_advance needs converge mode");
+    if (iters <= 0) throw Error(FIM_EINVAL, "iters must be positive");
+    const uint32_t l = run_iterations(c, iters, reset_stats != 0);
+    if (left) *left = (int32_t)l;
+  });
+}
+
+int fim_active(fim_ctx *c, int32_t *active) {
+  return guarded([&] {
+    if (!c || !active) throw Error(FIM_EINVAL, "null argument");
+    if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "fim_active needs converge mode");
+    HIPCHECK(hipSetDevice(c->dev));
+    k_inst_active<<<c->ninst, 256, 0, c->stream>>>(c->d_act, c->nsub, c->d_flag);
+    HIPCHECK(hipGetLastError());
+    std::vector<uint32_t> h(c->ninst);
+    HIPCHECK(hipMemcpyAsync(h.data(), c->d_flag, sizeof(uint32_t) * c->ninst, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (uint32_t i = 0; i < c->ninst; ++i) active[i] = (int32_t)h[i];
+  });
+}
+
+int fim_recycle(fim_ctx *c, int32_t n, const int32_t *insts) {
+  return guarded([&] {
+    if (!c) throw Error(FIM_EINVAL, "null context");
+    if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "fim_recycle needs converge mode");
+    const std::vector<uint32_t> v = checked_instances(c, n, insts);
+    if (v.empty()) return;
+    HIPCHECK(hipSetDevice(c->dev));
+    // only instances with nothing queued may be recycled
+    k_inst_active<<<c->ninst, 256, 0, c->stream>>>(c->d_act, c->nsub, c->d_flag);
+    HIPCHECK(hipGetLastError());
+    std::vector<uint32_t> h(c->ninst);
+    HIPCHECK(hipMemcpyAsync(h.data(), c->d_flag, sizeof(uint32_t) * c->ninst, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    std::vector<uint8_t> seen(c->ninst, 0);
+    for (uint32_t i : v) {
+      if (h[i]) throw Error(FIM_ESTATE, "instance " + std::to_string(i) + " still has queued work");
+      if (seen[i]++) throw Error(FIM_EINVAL, "instance listed twice");
+    }
+    HIPCHECK(hipMemcpyAsync(c->d_flag, v.data(), sizeof(uint32_t) * v.size(), hipMemcpyHostToDevice, c->stream));
+    const size_t total = ((size_t)c->ncopy + c->nsub) * v.size();
+    k_inst_reset<<<blocks_for(total, 256), 256, 0, c->stream>>>(c->d_flag, (uint32_t)v.size(), c->d_phi, c->d_dirty,
+                                                               c->d_wake, c->d_start, c->d_cnt + 48, c->ncopy,
+                                                               c->nsub);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(c->stream));
   });
 }
 

@@ -111,6 +111,9 @@ SIGNATURES = {
     "fim_reset": (C.c_int, [_vp]),
     "fim_seed": (C.c_int, [_vp, C.c_int32, _pi64, C.c_int64, C.c_double]),
     "fim_seed_batch": (C.c_int, [_vp, C.c_int32, _pi32, _pi64, _pi64, C.c_double]),
+    "fim_advance": (C.c_int, [_vp, C.c_int32, C.c_int32, _pi32]),
+    "fim_active": (C.c_int, [_vp, _pi32]),
+    "fim_recycle": (C.c_int, [_vp, C.c_int32, _pi32]),
     "fim_set_scaling": (C.c_int, [_vp, C.c_int32, _pf]),
     "fim_init_field": (C.c_int, [_vp, C.c_int32, _pf]),
     "fim_step": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _pi32]),
@@ -332,6 +335,25 @@ class Solver:
         it = C.c_int32()
         _check(lib().fim_run(self.h, max_iters, C.byref(it)))
         return it.value
+
+    def advance(self, iters, reset_stats=False):
+        """Runs at least `iters` global iterations (or until nothing is
+        queued); returns the queued work units left."""
+        left = C.c_int32()
+        _check(lib().fim_advance(self.h, iters, int(reset_stats), C.byref(left)))
+        return left.value
+
+    def active(self):
+        """Per instance: True while it still has queued work."""
+        a = np.zeros(self.n_instances, dtype=np.int32)
+        _check(lib().fim_active(self.h, _p(a, _pi32)))
+        return a.astype(bool)
+
+    def recycle(self, insts):
+        """Clears converged instances so they can be seeded again while the
+        others keep running."""
+        v = np.ascontiguousarray(insts, dtype=np.int32).reshape(-1)
+        _check(lib().fim_recycle(self.h, v.shape[0], _p(v, _pi32)))
 
     def download(self, inst=0):
         phi = np.zeros(self.nn)

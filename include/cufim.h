@@ -258,6 +258,19 @@ int fim_step(fim_ctx *c, double phi_start, double cur_time, double dt,
  * local FIM) for all instances until no submesh is active or max_iters is
  * reached (FIM_ENOCONV).  iters (may be NULL) receives the count. */
 int fim_run(fim_ctx *c, int32_t max_iters, int32_t *iters);
+/* Continuous batching (no counterpart in the reference, which solves one map
+ * per process).  fim_advance runs at least `iters` global iterations (whole
+ * captured graphs) or until nothing is queued, without FIM_ENOCONV; *left
+ * (may be NULL) receives the queued work units left.  Device counters
+ * accumulate until a call with reset_stats != 0.  fim_active writes, per
+ * instance, 1 if it still has queued work and 0 if its map is converged.
+ * fim_recycle clears converged instances (times to +inf) and restarts their
+ * time windows at the current iteration, so they can be seeded again
+ * (fim_seed / fim_seed_batch) while the others keep running: each map comes
+ * out bit-identical to a solve started on its own. */
+int fim_advance(fim_ctx *c, int32_t iters, int32_t reset_stats, int32_t *left);
+int fim_active(fim_ctx *c, int32_t *active);
+int fim_recycle(fim_ctx *c, int32_t n, const int32_t *insts);
 /* global phi (min over copies; >= 1e50 means unreached, the reference uses
  * DBL_MAX) and state (max over copies; converge mode reports FROZEN for
  * reached and FAR for unreached) of one instance; either may be NULL.

@@ -211,6 +211,40 @@ def test_seed_batch_equals_per_instance_seeds():
     b.close()
 
 
+def test_stream_maps_equal_lockstep_maps():
+    # continuous batching: 10 solves through 3 instances, recycled as they
+    # converge; every map equals the same solve run on its own, bit for bit
+    from fimpy.stream import stream_solve
+    cfg = cases.scaled(cases.SLAB, 8)
+    m, om = cases.box(cfg, seed=19)
+    p = F.Pack(m, F.PART_RCB, max_nn=256, seed=19)
+    sets = [cases.pacing_sites(cfg, k % 3 + 1, seed=70 + k) for k in range(10)]
+    got = {}
+    s = F.Solver(p, n_instances=3, window=200.0, local_iters=4)
+
+    def keep(j, inst):
+        got[j] = s.download(inst=inst)[0]
+
+    stream_solve(s, sets, on_map=keep)
+    s.close()
+    assert sorted(got) == list(range(10))
+    for j in (0, 4, 9):
+        one = F.Solver(p, n_instances=1, window=200.0, local_iters=4)
+        one.seed(sets[j], 0.0)
+        one.run()
+        ref = one.download()[0]
+        one.close()
+        assert np.array_equal(got[j], ref), j
+    ref, _ = O.fim_solve(om, sets[9], variant=O.LOCAL_EXACT)
+    _check_fp32(got[9], ref)
+    # an instance with queued work cannot be recycled
+    s = F.Solver(p, n_instances=2, window=200.0)
+    s.seed(sets[0], 0.0, inst=0)
+    with pytest.raises(F.FimError):
+        s.recycle([0])
+    s.close()
+
+
 def test_converge_deterministic_and_batched():
     cfg = cases.scaled(cases.SLAB, 8)
     m, om = cases.box(cfg, seed=7)
