@@ -17,10 +17,12 @@
 //    (k_conv_sync), one wave per work unit, pushes them to the other copies
 //    of the vertex with global atomicMin, marks those dirty and queues their
 //    submesh on the next list; a third (k_conv_plan) cuts the next list into
-//    work units.  The kernels read only their own list, so
-//    results are independent of scheduling (atomic min and "set" flags
-//    commute).  Iterations are captured in a hipGraph; the host reads one
-//    counter per graph launch.
+//    work units.  The kernels read only their own list, so results are
+//    independent of scheduling (atomic min and "set" flags commute).  Each
+//    instance's time window counts from its own start iteration, so a
+//    converged instance can be recycled and seeded again mid-run.
+//    Iterations are captured in a hipGraph; the host reads one counter per
+//    graph launch.
 //
 //  compat (FP64, reference local update): one workgroup per submesh runs the
 //    reference's per-window phases a-g (src/eikonal/cuda_fim.cu:1992-2334)
