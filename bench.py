@@ -79,12 +79,16 @@ def traffic_from_profiles(args):
 
 
 def kernel_source_digest():
-    """sha256 (first 16 hex digits) of the kernel sources; a PMC summary
-    applies only to the sources it was measured on."""
+    """sha256 (first 16 hex digits) of the kernel sources without // comments
+    and blank lines; a PMC summary applies only to the code it was measured
+    on."""
     import hashlib
     h = hashlib.sha256()
     for f in ("solver.hip", "local_solve.hpp"):
-        h.update(open(os.path.join(ROOT, "cufim-nvlink_amd", "csrc", f), "rb").read())
+        for line in open(os.path.join(ROOT, "cufim-nvlink_amd", "csrc", f)).read().splitlines():
+            code = re.sub(r"//.*$", "", line).rstrip()
+            if code:
+                h.update(code.encode() + b"\n")
     return h.hexdigest()[:16]
 
 
