@@ -106,7 +106,7 @@ def nn_leg(args, account, plain_s):
     val = WS.draw_solves(cfg, 64, 2002 + args.seed)
     test = WS.draw_solves(cfg, args.nn_test, 3003 + args.seed)
     kw = dict(batch=args.batch, window=args.window, profile=True, on_stats=account,
-              local_iters=args.local_iters)
+              local_iters=args.local_iters, block=args.block)
     nn = F.Surrogate(mesh, seed=args.seed, max_maps=args.nn_maps, max_batch=args.nn_batch)
     t_data = WS.build_dataset(nn, pack, train, unit=args.unit, **kw)
     log(f"[bench] surrogate dataset: {args.nn_maps} maps in {t_data:.2f}s")
@@ -124,7 +124,7 @@ def nn_leg(args, account, plain_s):
     delta = 1.25 * max(under, 0.0)
     # held-out solves: cold batched (timed), surrogate error on each map
     s = F.Solver(pack, n_instances=args.batch, window=args.window, unit=args.unit, profile=True,
-                 local_iters=args.local_iters)
+                 local_iters=args.local_iters, block=args.block)
     t_cold, cold_it, cold_ev, sur = 0.0, 0, 0, []
     for i in range(0, len(test), args.batch):
         chunk = test[i:i + args.batch]
@@ -195,8 +195,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--max-nn", type=int, default=1536)
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--max-nn", type=int, default=2816)
+    ap.add_argument("--block", type=int, default=512)
     ap.add_argument("--local-iters", type=int, default=4,
                     help="sweeps per activation before requeueing (0 = to local convergence)")
     ap.add_argument("--unit", type=int, default=2,
@@ -217,6 +217,7 @@ def main():
     ap.add_argument("--plain-max-nn", type=int, default=256)
     ap.add_argument("--plain-window", type=float, default=2000.0)
     ap.add_argument("--plain-local-iters", type=int, default=4)
+    ap.add_argument("--plain-block", type=int, default=256)
     ap.add_argument("--plain-waves", type=int, default=5,
                     help="waves per SIMD of the plain solve's compute kernel (0 = LDS bound)")
     ap.add_argument("--nn", action="store_true",
@@ -320,7 +321,7 @@ def main():
     # ---- plain single-solve GPU FIM (the denominator of the target) -----
     ppack = pack if args.plain_max_nn == args.max_nn else \
         F.Pack(mesh, F.PART_RCB, max_nn=args.plain_max_nn, seed=args.seed)
-    p1 = F.Solver(ppack, n_instances=1, block=args.block, profile=True, window=args.plain_window,
+    p1 = F.Solver(ppack, n_instances=1, block=args.plain_block, profile=True, window=args.plain_window,
                   local_iters=args.plain_local_iters, waves=args.plain_waves,
                   staged=args.staged)
     solve_batch(p1, sets[:1])
@@ -413,7 +414,8 @@ def main():
         "stream_maps_per_s": stream_maps_per_s,
         "plain_maps_per_s": 1.0 / plain_s,
         "plain_config": {"max_nn": args.plain_max_nn, "window_us": args.plain_window,
-                         "local_iters": args.plain_local_iters, "waves": args.plain_waves},
+                         "local_iters": args.plain_local_iters, "workgroup": args.plain_block,
+                         "waves": args.plain_waves},
         "batched_over_plain": maps_per_s * plain_s,
         "solves_per_s": maps_per_s,
         "elem_evals_per_s": tot["elem_updates"] / elapsed,

@@ -1668,7 +1668,10 @@ void launch_conv_compute(fim_ctx *c, const ConvArgs &a) {
     switch (c->block) {
       case 64: k_conv_band<64, 8, false><<<c->grid, 64, c->lds_conv, c->stream>>>(a); break;
       case 128: k_conv_band<128, 8, false><<<c->grid, 128, c->lds_conv, c->stream>>>(a); break;
-      case 512: k_conv_band<512, 8, false><<<c->grid, 512, c->lds_conv, c->stream>>>(a); break;
+      case 512:
+        if (c->band_pairs) k_conv_band<512, 8, true><<<c->grid, 512, c->lds_conv, c->stream>>>(a);
+        else k_conv_band<512, 8, false><<<c->grid, 512, c->lds_conv, c->stream>>>(a);
+        break;
       default: {
         const void *f = conv_kernel_ptr(false, 256, c->band_wpe, c->band_pairs);
         void *args[] = {const_cast<ConvArgs *>(&a)};
@@ -1684,7 +1687,8 @@ const void *conv_kernel_ptr(bool staged, int block, int wpe, bool pairs) {
   switch (block) {
     case 64: return staged ? (const void *)k_conv_compute<64, true> : (const void *)k_conv_band<64, 8, false>;
     case 128: return staged ? (const void *)k_conv_compute<128, true> : (const void *)k_conv_band<128, 8, false>;
-    case 512: return staged ? (const void *)k_conv_compute<512, true> : (const void *)k_conv_band<512, 8, false>;
+    case 512: return staged ? (const void *)k_conv_compute<512, true>
+                            : (pairs ? (const void *)k_conv_band<512, 8, true> : (const void *)k_conv_band<512, 8, false>);
     default:
       if (staged) return (const void *)k_conv_compute<256, true>;
       if (pairs) {
@@ -1910,16 +1914,26 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         // staged: element nodes + geometry; unstaged: element mark bitmasks
         const uint32_t o_geo = o_e2n + (o->staged ? al16(8 * (uint32_t)c->ne_max) : al16(4 * ne32));
         const uint32_t o_list = o_geo + (o->staged ? al16(24 * (uint32_t)c->ne_max) : al16(4 * ne32));
-        // the band kernel lists at most kElistCap elements per pass
-        const uint32_t elcap = o->staged ? (uint32_t)c->ne_max
-                                         : std::min<uint32_t>((uint32_t)c->ne_max, (uint32_t)kElistCap);
+        // workgroups of 256 and 512 evaluate (element, target) pairs when
+        // local element ids fit 14 bits; the pair rings live in the source list
+        c->band_pairs = !o->staged && (c->block == 256 || c->block == 512) && c->ne_max < (1 << 14);
+        const uint32_t vlist_bytes =
+            o->staged ? 0u
+                      : al16(std::max<uint32_t>(2 * nn64, c->band_pairs ? (uint32_t)(c->block / 64) * 2u * kPairRing : 0u));
+        const uint32_t tail_bytes = vlist_bytes + 32 + (o->staged ? 0u : 3u * al16(4 * (nn64 / 32)));
+        // The band kernel lists at most elcap elements per pass: at least
+        // kElistCap, more if the LDS left over by the workgroups the wave
+        // slots admit (8 waves per SIMD) holds more
+        uint32_t elcap = (uint32_t)c->ne_max;
+        if (!o->staged) {
+          const uint32_t wg_slots = std::max<uint32_t>(1u, 32u / ((uint32_t)c->block / 64u));
+          const uint32_t budget = (160u * 1024u / wg_slots) & ~15u;
+          const uint32_t base = o_list + tail_bytes;
+          const uint32_t room = budget > base ? ((budget - base) / 2u) & ~7u : 0u;
+          elcap = std::min<uint32_t>((uint32_t)c->ne_max, std::max<uint32_t>((uint32_t)kElistCap, room));
+        }
         const uint32_t o_vlist = o_list + al16(2 * elcap);
-        // workgroups of 256 evaluate (element, target) pairs when local
-        // element ids fit 14 bits; the pair rings live in the source list
-        c->band_pairs = !o->staged && c->block == 256 && c->ne_max < (1 << 14);
-        const uint32_t o_misc =
-            o_vlist + (o->staged ? 0u
-                                 : al16(std::max<uint32_t>(2 * nn64, c->band_pairs ? 4u * 2u * kPairRing : 0u)));
+        const uint32_t o_misc = o_vlist + vlist_bytes;
         const uint32_t o_wrt = o_misc + 32;  // misc: 8 words
         // unstaged: interface-copy bits and entry counts (2 x nn64/32 words)
         const uint32_t o_ifc = o_wrt + (o->staged ? 0u : al16(4 * (nn64 / 32)));

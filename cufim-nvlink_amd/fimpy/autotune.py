@@ -4,8 +4,8 @@ the compute kernel's register budget for a mesh and batch size by measuring
 short batched solves on the GPU.
 
 Search: coordinate descent from the defaults over
-    max_nn      {256, 512, 1024, 1536, 2048}  vertex copies per submesh
-    block       {64, 128, 256}                workgroup size
+    max_nn      {256, 512, 1024, 1536, 2048, 2816}  vertex copies per submesh
+    block       {64, 128, 256, 512}                 workgroup size
     local_iters {0, 1, 2, 4, 8}               sweeps per activation (0 = to local convergence)
     window      {500, 1000, 2000, 4000}       microseconds per global iteration
     waves       {0, 5}                        waves per SIMD of the compute kernel (0 = LDS bound)
@@ -25,13 +25,13 @@ import numpy as np
 from . import PART_RCB, Mesh, Pack, Solver
 
 SPACE = {
-    "max_nn": [256, 512, 1024, 1536, 2048],
-    "block": [64, 128, 256],
+    "max_nn": [256, 512, 1024, 1536, 2048, 2816],
+    "block": [64, 128, 256, 512],
     "local_iters": [0, 1, 2, 4, 8],
     "window": [500.0, 1000.0, 2000.0, 4000.0],
     "waves": [0, 5],
 }
-DEFAULT = {"max_nn": 1536, "block": 256, "local_iters": 4, "window": 2000.0, "waves": 0}
+DEFAULT = {"max_nn": 2816, "block": 512, "local_iters": 4, "window": 2000.0, "waves": 0}
 
 
 def signature(mesh: Mesh, n_instances: int) -> str:

@@ -49,13 +49,13 @@ def _note(on_stats, st):
 
 
 def build_dataset(nn: Surrogate, pack: Pack, solves, batch=256, window=2000.0, unit=0,
-                  profile=False, on_stats=None, local_iters=0):
+                  profile=False, on_stats=None, local_iters=0, block=256):
     """Solves every solve cold in batches and appends the maps to the
     surrogate's dataset.  Returns seconds spent.  on_stats(stats) is called
     after every batch (profile=True times every compute launch)."""
     t0 = time.perf_counter()
     s = Solver(pack, n_instances=batch, window=window, unit=unit, profile=profile,
-               local_iters=local_iters)
+               local_iters=local_iters, block=block)
     try:
         for i, chunk in _chunks(solves, batch):
             _note(on_stats, cold_batch(s, chunk + [chunk[0]] * (batch - len(chunk))))
@@ -66,11 +66,12 @@ def build_dataset(nn: Surrogate, pack: Pack, solves, batch=256, window=2000.0, u
 
 
 def surrogate_errors(nn: Surrogate, pack: Pack, solves, batch=256, window=2000.0, profile=False,
-                     on_stats=None, local_iters=0):
+                     on_stats=None, local_iters=0, block=256):
     """Per-solve (max underestimate, max |error|, mean |error|, max T) of the
     bare surrogate against the converged FIM maps, in us."""
     out = []
-    s = Solver(pack, n_instances=batch, window=window, profile=profile, local_iters=local_iters)
+    s = Solver(pack, n_instances=batch, window=window, profile=profile, local_iters=local_iters,
+               block=block)
     try:
         for i, chunk in _chunks(solves, batch):
             _note(on_stats, cold_batch(s, chunk + [chunk[0]] * (batch - len(chunk))))
@@ -109,13 +110,13 @@ def judge(solver: Solver, n, tol=1e-4):
 
 def warm_solve_all(nn: Surrogate, pack: Pack, solves, delta, batch=256, window=2000.0, unit=0,
                    all_active=True, tol=1e-4, judge_maps=True, profile=False, on_stats=None,
-                   local_iters=0):
+                   local_iters=0, block=256):
     """Warm-started batched solves with the support check as judge and a cold
     re-solve of the rejected ones.  Returns a dict of times and counts."""
     s = Solver(pack, n_instances=batch, window=window, unit=unit, profile=profile,
-               local_iters=local_iters)
+               local_iters=local_iters, block=block)
     fb = Solver(pack, n_instances=batch, window=window, unit=unit, profile=profile,
-                local_iters=local_iters)
+                local_iters=local_iters, block=block)
     rep = dict(maps=0, infer_s=0.0, fim_s=0.0, judge_s=0.0, fallback_s=0.0, fallbacks=0,
                global_iters=0, activations=0, elem_updates=0)
     try:

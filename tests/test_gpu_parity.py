@@ -140,8 +140,8 @@ def test_converge_workgroup_shapes_agree(block, staged):
 def test_converge_long_element_lists_and_pair_path_agree():
     # 2048-copy submeshes seeded on their whole bottom faces: sweeps mark more
     # elements than one element-list pass holds, so they run in several
-    # passes.  Workgroups of 256 evaluate (element, target) pairs, of 128 one
-    # element per lane: the same sweeps, so bit-identical maps, and both
+    # passes.  Workgroups of 256 and 512 evaluate (element, target) pairs, of
+    # 128 one element per lane: the same sweeps, so bit-identical maps, all
     # within tolerance of the FP64 oracle.
     cfg = cases.scaled(cases.SLAB, 4)
     m, om = cases.box(cfg, seed=17)
@@ -151,13 +151,13 @@ def test_converge_long_element_lists_and_pair_path_agree():
     src = np.nonzero(xyz[:, 2] < xyz[:, 2].min() + 0.2 * cfg["lz"] / cfg["nz"])[0]
     assert len(src) > 1000  # a bottom layer of ~2,400 tets per submesh is marked at once
     maps = {}
-    for block in (128, 256):
+    for block in (128, 256, 512):
         s = F.Solver(p, block=block, window=0.0, local_iters=4)
         s.seed(src, 0.0)
         s.run()
         maps[block] = s.download()[0]
         s.close()
-    assert np.array_equal(maps[128], maps[256])
+    assert np.array_equal(maps[128], maps[256]) and np.array_equal(maps[128], maps[512])
     ref, _ = O.fim_solve(om, src, variant=O.LOCAL_EXACT)
     _check_fp32(maps[256], ref)
 
