@@ -212,13 +212,14 @@ def main():
                     help="stage whole submeshes in LDS (default: element data from L2)")
     ap.add_argument("--window", type=float, default=2000.0,
                     help="time window per global iteration in us (0 = plain FIM order)")
-    # plain single-solve GPU FIM: its own best setting (fewer, smaller work
-    # items want smaller submeshes and a finer window)
-    ap.add_argument("--plain-max-nn", type=int, default=256)
+    # plain single-solve GPU FIM: its own best setting, found by a sweep of
+    # submesh size (256..2816 copies), workgroup (128/256/512), sweeps (2..8)
+    # and window (1..4 ms): one map wants more, smaller submeshes
+    ap.add_argument("--plain-max-nn", type=int, default=1024)
     ap.add_argument("--plain-window", type=float, default=2000.0)
-    ap.add_argument("--plain-local-iters", type=int, default=4)
-    ap.add_argument("--plain-block", type=int, default=256)
-    ap.add_argument("--plain-waves", type=int, default=5,
+    ap.add_argument("--plain-local-iters", type=int, default=3)
+    ap.add_argument("--plain-block", type=int, default=512)
+    ap.add_argument("--plain-waves", type=int, default=0,
                     help="waves per SIMD of the plain solve's compute kernel (0 = LDS bound)")
     ap.add_argument("--nn", action="store_true",
                     help="also measure the surrogate warm start (adds about a minute)")
