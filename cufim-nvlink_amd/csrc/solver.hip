@@ -1120,7 +1120,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
 
 struct SyncArgs {
   const SubInfo *sub;
-  const uint32_t *if_dsp;
+  const uint4 *if_fo;  // per entry: first other copy, its submesh, then [z, w) in if_os
   const uint2 *if_os;  // per other copy: (copy, its submesh)
   const uint2 *prec;
   const uint32_t *pcnt;
@@ -1133,19 +1133,21 @@ struct SyncArgs {
   uint32_t nsub, ncopy, ninst, n_if, unit, maxch;
 };
 
-// interface push: one wave per work unit of the iteration's list, a lane per
-// push record of its items; min over copies, order independent
+// interface push: one wave per item (instance, submesh) of the iteration's
+// work units, a lane per push record; min over copies, order independent.
+// The kernel waits on memory, so it runs on its own full-occupancy grid.
 __global__ __launch_bounds__(256) void k_conv_sync(SyncArgs a) {
   // the compute kernel of this iteration has finished reading the counter
   if (blockIdx.x == 0 && threadIdx.x == 0) *a.iter += 1u;
   const uint32_t lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
-  const uint32_t U = a.cur.cnt[0], nw = gridDim.x * wpb;
-  for (uint32_t u = blockIdx.x * wpb + (threadIdx.x >> 6); u < U; u += nw) {
+  const uint32_t NI = a.cur.cnt[0] * a.unit, nw = gridDim.x * wpb;
+  for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < NI; w += nw) {
+    const uint32_t u = w / a.unit, q = w - u * a.unit;
     const uint32_t pu = a.cur.plan[u];
     const uint32_t s = pu / a.maxch, ch = pu - s * a.maxch;
     const uint32_t n_in = min(a.unit, a.cur.bcnt[s] - ch * a.unit);
     const uint32_t ioff = a.sub[s].ioff;
-    for (uint32_t q = 0; q < n_in; ++q) {
+    if (q < n_in) {
       const uint32_t inst = a.cur.bins[(size_t)s * a.ninst + ch * a.unit + q];
       const uint32_t n = a.pcnt[(size_t)inst * a.nsub + s];
       const uint2 *rec = a.prec + (size_t)inst * a.n_if + ioff;
@@ -1153,8 +1155,10 @@ __global__ __launch_bounds__(256) void k_conv_sync(SyncArgs a) {
       for (uint32_t r = lane; r < n; r += 64) {
         const uint2 e = rec[r];
         const uint32_t v = e.y;
-        for (uint32_t k = a.if_dsp[e.x]; k < a.if_dsp[e.x + 1]; ++k) {
-          const uint2 o = a.if_os[k];
+        // most entries have one other copy: the entry's record holds it
+        const uint4 fo = a.if_fo[e.x];
+        uint2 o = make_uint2(fo.x, fo.y);
+        for (uint32_t k = fo.z;;) {
           const size_t pe = (size_t)inst * a.nsub + o.y;
           uint32_t *p = reinterpret_cast<uint32_t *>(a.phi + ib + o.x);
           uint32_t *wk = reinterpret_cast<uint32_t *>(a.wake + pe);
@@ -1173,6 +1177,8 @@ __global__ __launch_bounds__(256) void k_conv_sync(SyncArgs a) {
             if (v < wcur) __hip_atomic_fetch_min(wk, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (queued == 0u) enqueue(a.nxt, a.act, inst, o.y, a.nsub, a.ninst);
           }
+          if (k >= fo.w) break;
+          o = a.if_os[k++];
         }
       }
     }
@@ -1604,7 +1610,7 @@ struct fim_ctx {
   uint32_t *d_c2g = nullptr, *d_c2s = nullptr, *d_g2c_dsp = nullptr, *d_g2c = nullptr, *d_grp = nullptr;
   uint32_t ngrp = 0;
   uint16_t *d_if_l = nullptr;
-  uint32_t *d_if_dsp = nullptr;
+  uint4 *d_if_fo = nullptr;
   uint2 *d_if_os = nullptr;
   uint32_t *d_n2e_dsp = nullptr, *d_n2n_dsp = nullptr;
   uint16_t *d_n2e_le = nullptr, *d_n2n_lv = nullptr;
@@ -1646,7 +1652,7 @@ struct fim_ctx {
   SyncArgs sa{};
   CompatArgs xa{};
   uint32_t lds_conv = 0, lds_compat = 0;
-  int grid = 0, block = 256, graph_iters = 8;
+  int grid = 0, sync_grid = 0, block = 256, graph_iters = 8;
   int band_wpe = 8;  // waves per SIMD the band kernel's registers are sized for
   bool band_pairs = false;  // band kernel evaluates (element, target) pairs
   fim_stats_t stats{};
@@ -1752,7 +1758,7 @@ void enqueue_iteration(fim_ctx *c, int k, bool timed) {
   SyncArgs s = c->sa;
   s.cur = c->wl[p];
   s.nxt = c->wl[q];
-  k_conv_sync<<<c->grid, 256, 0, c->stream>>>(s);
+  k_conv_sync<<<c->sync_grid, 256, 0, c->stream>>>(s);
   HIPCHECK(hipGetLastError());
   launch_plan(c, q, p);
 }
@@ -1983,11 +1989,17 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
           c->d_adj = dupload(adj4, ow);
         }
         c->d_if_l = dupload(p.if_l, ow);
-        c->d_if_dsp = dupload(p.if_dsp, ow);
         {
           std::vector<uint2> os(p.if_other.size());
           for (size_t k = 0; k < os.size(); ++k) os[k] = make_uint2(p.if_other[k], p.c2s[p.if_other[k]]);
+          std::vector<uint4> fo(p.if_l.size());
+          for (size_t e = 0; e < fo.size(); ++e) {
+            const uint32_t k0 = p.if_dsp[e], k1 = p.if_dsp[e + 1];
+            if (k1 <= k0) throw Error(FIM_EINVAL, "interface entry without another copy");
+            fo[e] = make_uint4(os[k0].x, os[k0].y, k0 + 1, k1);
+          }
           c->d_if_os = dupload(os, ow);
+          c->d_if_fo = dupload(fo, ow);
         }
         const size_t ncs = (size_t)c->ninst * c->ncopy, nss = (size_t)c->ninst * c->nsub;
         c->d_phi = dalloc<float>(ncs, ow);
@@ -2054,6 +2066,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         per_cu = std::max(per_cu, 1);
         c->grid = o->grid > 0 ? o->grid : ncu * per_cu;
         c->grid = std::max(c->grid, (int)kXcd);  // every XCD eighth needs a workgroup
+        c->sync_grid = ncu * 8;  // eight workgroups of 256 fill a CU's wave slots
         c->lds_compat = 0;
         c->ca = ConvArgs{c->d_sub, c->d_e2n, c->d_geo32, c->d_hgt, c->d_ereg, c->d_rscale, c->d_phi,
                          c->d_dirty, c->d_act, c->d_wake, c->d_if_l, c->d_ifm, c->d_prec,
@@ -2063,7 +2076,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
                          (float)o->window, o->local_iters, c->unit, c->maxch, (uint32_t)p.if_l.size(),
                          c->wps, elcap, o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc,
                          o_wrt, o_ifc};
-        c->sa = SyncArgs{c->d_sub, c->d_if_dsp, c->d_if_os, c->d_prec, c->d_pcnt,
+        c->sa = SyncArgs{c->d_sub, c->d_if_fo, c->d_if_os, c->d_prec, c->d_pcnt,
                          c->d_phi, c->d_dirty, c->d_act, c->d_wake, WorkList{}, WorkList{},
                          c->d_cnt + 48, c->nsub, c->ncopy, c->ninst, (uint32_t)p.if_l.size(),
                          c->unit, c->maxch};
