@@ -95,6 +95,38 @@ def kernel_source_digest():
 NN_PARAMS = 333569
 
 
+def compat_leg(args, mesh, sites):
+    """The reference's time-window algorithm (compat mode, FP64) on the slab:
+    the fim_step loop timed like the reference's solve (step loop only,
+    cuda_fim.cu:2424-2447), one pacing set, own partition.  Returns the
+    report and what the CPU side needs to replay the same solve."""
+    pack = F.Pack(mesh, F.PART_RCB, max_nn=args.compat_max_nn, seed=args.seed)
+    s = F.Solver(pack, mode=F.MODE_COMPAT, local=F.LOCAL_REF)
+    fw = mesh.front_width()
+    stims = [(1, 0.0, [int(v) for v in sites])]
+    best = None
+    for rep in range(2):  # the first run warms the device up
+        s.reset()
+        s.seed(sites, 0.0)
+        cur = 0.0
+        t0 = time.perf_counter()
+        for _ in range(args.compat_steps):
+            ps = cur
+            cur += args.compat_dt
+            s.step(ps, cur, args.compat_dt, fw)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    phi, st = s.download()
+    s.close()
+    e2s, _ = pack.partition()
+    rep = {"max_nn": args.compat_max_nn, "submeshes": pack.info["nsub"], "dt_us": args.compat_dt,
+           "steps": args.compat_steps, "gpu_ms_per_step": 1e3 * best / args.compat_steps,
+           "finite": int(np.sum(phi < 1e50))}
+    log(f"[bench] compat: {args.compat_steps} steps of {args.compat_dt:.0f} us, "
+        f"{rep['gpu_ms_per_step']:.3f} ms per step on the GPU")
+    return rep, (phi, st, e2s, stims, fw)
+
+
 def nn_leg(args, account, plain_s):
     """Surrogate warm start on the slab with quadrant speed factors; returns
     the report dict."""
@@ -221,6 +253,11 @@ def main():
     ap.add_argument("--plain-block", type=int, default=512)
     ap.add_argument("--plain-waves", type=int, default=0,
                     help="waves per SIMD of the plain solve's compute kernel (0 = LDS bound)")
+    ap.add_argument("--compat-steps", type=int, default=0,
+                    help="also time the reference time-window algorithm (compat mode, FP64) for this "
+                         "many steps on the slab, GPU and host oracle (about a minute)")
+    ap.add_argument("--compat-dt", type=float, default=2000.0)
+    ap.add_argument("--compat-max-nn", type=int, default=128)
     ap.add_argument("--nn", action="store_true",
                     help="also measure the surrogate warm start (adds about a minute)")
     ap.add_argument("--nn-maps", type=int, default=4096)
@@ -338,6 +375,11 @@ def main():
     # ---- surrogate warm start ---------------------------------------------
     nnrep = nn_leg(args, account, plain_s) if args.nn else None
 
+    # ---- reference time-window algorithm (compat mode) ---------------------
+    compat, compat_replay = (None, None)
+    if args.compat_steps > 0:
+        compat, compat_replay = compat_leg(args, mesh, sets[0])
+
     # ---- accuracy and CPU baseline (FP64 oracle, one map per host thread) --
     max_dt = None
     cpu = None
@@ -369,6 +411,19 @@ def main():
         times = [dt for _, dt in res]
         log(f"[bench] oracle {n_cpu} maps in {wall:.2f}s (per map {min(times):.1f}-{max(times):.1f}s), "
             f"max|dT| over them = {max_dt:.3f} us")
+        if compat_replay is not None and not args.no_cpu_baseline:
+            # the same compat solve on one host thread, same partition
+            cphi, cst, e2s, stims, fw = compat_replay
+            omc = O.Mesh(xyz, e2n, fib=fib, vel=vel, atags=e2s.astype(np.int32))
+            t0 = time.perf_counter()
+            ophi, ost, _ = O.compat_solve(omc, stims, args.compat_dt, args.compat_steps, front_width=fw)
+            tc = time.perf_counter() - t0
+            compat["cpu_ms_per_step"] = 1e3 * tc / args.compat_steps
+            compat["cpu_cores"] = 1
+            compat["gpu_over_cpu"] = compat["cpu_ms_per_step"] / compat["gpu_ms_per_step"]
+            compat["bit_identical"] = bool(np.array_equal(cphi, ophi) and np.array_equal(cst, ost))
+            log(f"[bench] compat on one host thread: {compat['cpu_ms_per_step']:.2f} ms per step "
+                f"(setup included), bit-identical {compat['bit_identical']}")
         cpu = {"value": n_cpu / float(np.mean(times)), "unit": "maps/s", "cores": n_cpu, "kind": "port",
                "sample": f"{n_cpu} maps of the same slab and pacing sets (instances 0..{n_cpu - 1}), one "
                          f"per host thread, FP64 label-correcting fixed point of the same local update; "
@@ -437,6 +492,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "nn_warm_start": nnrep,
+        "compat": compat,
     }
     print(json.dumps(out), flush=True)
 
