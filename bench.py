@@ -22,6 +22,11 @@ fallback) against the same 512 solved cold.  Its numbers are reported in
 `nn_warm_start`; the headline value is the batched solver alone, which is
 the faster of the two on this workload (the warm start is 3x slower).
 
+With --compat-steps N the reference's time-window algorithm (compat mode,
+FP64) runs N windows of one map on the same slab, on the GPU and then on one
+host thread with the oracle, and the two maps are compared bit for bit
+(`compat` in the line).
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
 from __future__ import annotations
