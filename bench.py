@@ -135,6 +135,7 @@ def compat_leg(args, mesh, sites):
 def nn_leg(args, account, plain_s):
     """Surrogate warm start on the slab with quadrant speed factors; returns
     the report dict."""
+    nb = args.nn_solve_batch  # instances per batched solve of this leg
     cfg = W.SLAB
     mesh = W.box(cfg, args.seed)
     mesh.set_regions(W.quadrant_regions(mesh))
@@ -142,7 +143,7 @@ def nn_leg(args, account, plain_s):
     train = WS.draw_solves(cfg, args.nn_maps, 1001 + args.seed)
     val = WS.draw_solves(cfg, 64, 2002 + args.seed)
     test = WS.draw_solves(cfg, args.nn_test, 3003 + args.seed)
-    kw = dict(batch=args.batch, window=args.window, profile=True, on_stats=account,
+    kw = dict(batch=nb, window=args.window, profile=True, on_stats=account,
               local_iters=args.local_iters, block=args.block)
     nn = F.Surrogate(mesh, seed=args.seed, max_maps=args.nn_maps, max_batch=args.nn_batch)
     t_data = WS.build_dataset(nn, pack, train, unit=args.unit, **kw)
@@ -160,12 +161,12 @@ def nn_leg(args, account, plain_s):
     under = max(e[0] for e in errs)
     delta = 1.25 * max(under, 0.0)
     # held-out solves: cold batched (timed), surrogate error on each map
-    s = F.Solver(pack, n_instances=args.batch, window=args.window, unit=args.unit, profile=True,
+    s = F.Solver(pack, n_instances=nb, window=args.window, unit=args.unit, profile=True,
                  local_iters=args.local_iters, block=args.block)
     t_cold, cold_it, cold_ev, sur = 0.0, 0, 0, []
-    for i in range(0, len(test), args.batch):
-        chunk = test[i:i + args.batch]
-        full = chunk + [chunk[0]] * (args.batch - len(chunk))
+    for i in range(0, len(test), nb):
+        chunk = test[i:i + nb]
+        full = chunk + [chunk[0]] * (nb - len(chunk))
         t0 = time.perf_counter()
         st = WS.cold_batch(s, full)
         t_cold += time.perf_counter() - t0
@@ -231,7 +232,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=2048,
+                    help="maps per step (instances of the batched solver)")
     ap.add_argument("--max-nn", type=int, default=2816)
     ap.add_argument("--block", type=int, default=512)
     ap.add_argument("--local-iters", type=int, default=4,
@@ -268,6 +270,8 @@ def main():
     ap.add_argument("--nn-maps", type=int, default=4096)
     ap.add_argument("--nn-steps", type=int, default=4000)
     ap.add_argument("--nn-batch", type=int, default=65536)
+    ap.add_argument("--nn-solve-batch", type=int, default=256,
+                    help="instances per batched solve in the surrogate leg")
     ap.add_argument("--nn-test", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0,
