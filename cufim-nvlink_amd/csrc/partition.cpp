@@ -189,7 +189,7 @@ int64_t converge_lds_bytes(int32_t nn_max, int32_t ne_max) {
   // same carve as the default converge-mode kernel: vertex times and
   // candidate minima (2 x f32 per vertex, rounded to 64), four vertex
   // bitmasks, two element bitmasks, element list (u16), source list (u16),
-  // 8 counters, interface-copy bits and entry counts
+  // 8 counters and 4 region factors, interface-copy bits and entry counts
   auto al = [](int64_t x) { return (x + 15) & ~int64_t(15); };
   const int64_t nn64 = (nn_max + 63) & ~int64_t(63);
   const int64_t ne32 = (ne_max + 31) / 32;
@@ -197,7 +197,7 @@ int64_t converge_lds_bytes(int32_t nn_max, int32_t ne_max) {
   // four waves of a workgroup of 256: at least 4 x 320 u16)
   return 2 * al(4 * nn64) + 4 * al(4 * (nn64 / 32)) + 2 * al(4 * ne32) +
          al(2LL * std::min<int64_t>(ne_max, kElistCap)) +
-         al(std::max<int64_t>(2 * nn64, ne_max < (1 << 14) ? 2560 : 0)) + 32 + 2 * al(4 * (nn64 / 32));
+         al(std::max<int64_t>(2 * nn64, ne_max < (1 << 14) ? 2560 : 0)) + 48 + 2 * al(4 * (nn64 / 32));
 }
 
 Pack build_pack(const Mesh &m, const Partition &p, int32_t source) {

@@ -715,8 +715,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
   uint16_t *elist = reinterpret_cast<uint16_t *>(lds + a.o_list);
   uint16_t *vlist = reinterpret_cast<uint16_t *>(lds + a.o_vlist);
   // misc: [0] listed elements, [2] listed sources, [3] distinct elements,
-  // [4] work cursor, [5] earliest deferred time, [6] push records
+  // [4] work cursor, [5] earliest deferred time, [6] push records,
+  // [8..11] the instance's region factors
   uint32_t *misc = reinterpret_cast<uint32_t *>(lds + a.o_misc);
+  const float *rsl = reinterpret_cast<const float *>(misc + 8);
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
   UnitCursor cursor(a.cur, &misc[4], a.ninst, a.unit, a.maxch);
@@ -752,6 +754,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
       misc[5] = __float_as_uint(F_INF);
       misc[6] = 0u;
     }
+    if (tid < 4) misc[8 + tid] = __float_as_uint(a.rscale[4 * (size_t)inst + tid]);
     __syncthreads();
     // stage vertex times; dirty flags split into sources and deferred
     for (uint32_t l0 = 0; l0 < nv64; l0 += BS) {
@@ -790,7 +793,6 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
         wave_append(d, l, vlist, &misc[2]);
       }
     }
-    const float *rs = a.rscale + 4 * (size_t)inst;
     __syncthreads();
 
     uint32_t it = 0;
@@ -861,7 +863,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
                 e = elist[i];
                 const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
                 const uint4 r0 = rec[0], r2 = rec[2];
-                const float hs = csqrt(rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)]);
+                const float hs = csqrt(rsl[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)]);
                 const float h[4] = {__uint_as_float(r2.x) * hs, __uint_as_float(r2.y) * hs,
                                     __uint_as_float(r2.z) * hs, __uint_as_float(r2.w) * hs};
                 const uint32_t nd[4] = {r0.x & 0x7fffu, (r0.x >> 16) & 0x7fffu, r0.y & 0xffffu,
@@ -903,7 +905,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
                 const uint32_t e = code & 0x3fffu, T = code >> 14;
                 const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
                 const uint4 r0 = rec[0], r1 = rec[1];
-                const float sc = rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
+                const float sc = rsl[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
                 const float g[6] = {__uint_as_float(r0.z) * sc, __uint_as_float(r0.w) * sc,
                                     __uint_as_float(r1.x) * sc, __uint_as_float(r1.y) * sc,
                                     __uint_as_float(r1.z) * sc, __uint_as_float(r1.w) * sc};
@@ -954,7 +956,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
             const uint32_t e = elist[i];
             const uint4 *rec = a.erec + 3 * ((size_t)si.eoff + e);
             const uint4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-            const float sc = rs[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
+            const float sc = rsl[((r0.x >> 15) & 1u) | ((r0.x >> 30) & 2u)];
             const float g[6] = {__uint_as_float(r0.z) * sc, __uint_as_float(r0.w) * sc,
                                 __uint_as_float(r1.x) * sc, __uint_as_float(r1.y) * sc,
                                 __uint_as_float(r1.z) * sc, __uint_as_float(r1.w) * sc};
@@ -1926,7 +1928,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         const uint32_t vlist_bytes =
             o->staged ? 0u
                       : al16(std::max<uint32_t>(2 * nn64, c->band_pairs ? (uint32_t)(c->block / 64) * 2u * kPairRing : 0u));
-        const uint32_t tail_bytes = vlist_bytes + 32 + (o->staged ? 0u : 3u * al16(4 * (nn64 / 32)));
+        const uint32_t tail_bytes = vlist_bytes + 48 + (o->staged ? 0u : 3u * al16(4 * (nn64 / 32)));
         // The band kernel lists at most elcap elements per pass: at least
         // kElistCap, more if the LDS left over by the workgroups the wave
         // slots admit (8 waves per SIMD) holds more
@@ -1940,7 +1942,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         }
         const uint32_t o_vlist = o_list + al16(2 * elcap);
         const uint32_t o_misc = o_vlist + vlist_bytes;
-        const uint32_t o_wrt = o_misc + 32;  // misc: 8 words
+        const uint32_t o_wrt = o_misc + 48;  // misc: 8 words, 4 region factors
         // unstaged: interface-copy bits and entry counts (2 x nn64/32 words)
         const uint32_t o_ifc = o_wrt + (o->staged ? 0u : al16(4 * (nn64 / 32)));
         c->lds_conv = o_ifc + (o->staged ? 0u : 2u * al16(4 * (nn64 / 32)));
