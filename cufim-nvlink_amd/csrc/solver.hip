@@ -1874,7 +1874,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
       HIPCHECK(hipEventCreate(&c->ev1));
       if (o->mode != FIM_MODE_CONVERGE && o->mode != FIM_MODE_COMPAT)
         throw Error(FIM_EINVAL, "unknown mode");
-      c->block = o->block ? o->block : 256;
+      c->block = o->block ? o->block : (o->mode == FIM_MODE_CONVERGE ? 512 : 256);
       if (c->block != 64 && c->block != 128 && c->block != 256 && c->block != 512)
         throw Error(FIM_EINVAL, "block must be 64, 128, 256 or 512");
       if (o->waves < 0 || o->waves > 8) throw Error(FIM_EINVAL, "waves must be 0..8");

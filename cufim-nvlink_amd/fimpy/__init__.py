@@ -287,7 +287,7 @@ class Pack:
 
 class Solver:
     def __init__(self, pack: Pack, mode=MODE_CONVERGE, local=LOCAL_EXACT, n_instances=1, device=0,
-                 block=256, local_iters=0, graph_iters=0, grid=0, eps_rel=0.0, apd=0.0,
+                 block=0, local_iters=0, graph_iters=0, grid=0, eps_rel=0.0, apd=0.0,
                  profile=False, window=0.0, staged=False, unit=0, waves=0):
         self.pack = pack
         self.nn = pack.info["nn_global"]

@@ -49,7 +49,7 @@ def _note(on_stats, st):
 
 
 def build_dataset(nn: Surrogate, pack: Pack, solves, batch=256, window=2000.0, unit=0,
-                  profile=False, on_stats=None, local_iters=0, block=256):
+                  profile=False, on_stats=None, local_iters=0, block=0):
     """Solves every solve cold in batches and appends the maps to the
     surrogate's dataset.  Returns seconds spent.  on_stats(stats) is called
     after every batch (profile=True times every compute launch)."""
@@ -66,7 +66,7 @@ def build_dataset(nn: Surrogate, pack: Pack, solves, batch=256, window=2000.0, u
 
 
 def surrogate_errors(nn: Surrogate, pack: Pack, solves, batch=256, window=2000.0, profile=False,
-                     on_stats=None, local_iters=0, block=256):
+                     on_stats=None, local_iters=0, block=0):
     """Per-solve (max underestimate, max |error|, mean |error|, max T) of the
     bare surrogate against the converged FIM maps, in us."""
     out = []
@@ -110,7 +110,7 @@ def judge(solver: Solver, n, tol=1e-4):
 
 def warm_solve_all(nn: Surrogate, pack: Pack, solves, delta, batch=256, window=2000.0, unit=0,
                    all_active=True, tol=1e-4, judge_maps=True, profile=False, on_stats=None,
-                   local_iters=0, block=256):
+                   local_iters=0, block=0):
     """Warm-started batched solves with the support check as judge and a cold
     re-solve of the rejected ones.  Returns a dict of times and counts."""
     s = Solver(pack, n_instances=batch, window=window, unit=unit, profile=profile,

@@ -185,7 +185,8 @@ typedef struct {
                             unless set to EXACT)                            */
   int32_t n_instances;   /* independent solves on the pack (converge mode)   */
   int32_t device;        /* HIP device ordinal                               */
-  int32_t block;         /* workgroup size: 64, 128, 256, 512 (0 = 256)      */
+  int32_t block;         /* workgroup size: 64, 128, 256, 512 (0 = 512 in    */
+                         /* converge mode, 256 in compat mode)               */
   int32_t local_iters;   /* max local iterations per activation (0 = until
                             locally converged)                              */
   int32_t graph_iters;   /* global iterations per captured graph, rounded up

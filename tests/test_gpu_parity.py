@@ -173,7 +173,7 @@ def test_converge_register_variants_agree(max_nn):
     ref, _ = O.fim_solve(om, srcs[0], variant=O.LOCAL_EXACT)
     base = None
     for waves in (0, 5, 6, 7, 8):
-        s = F.Solver(p, n_instances=3, window=50.0, local_iters=4, waves=waves)
+        s = F.Solver(p, n_instances=3, window=50.0, local_iters=4, waves=waves, block=256)
         for i, src in enumerate(srcs):
             s.seed(src, 0.0, inst=i)
         s.run()
