@@ -8,7 +8,8 @@ Search: coordinate descent from the defaults over
     block       {64, 128, 256, 512}                 workgroup size
     local_iters {0, 1, 2, 4, 8}               sweeps per activation (0 = to local convergence)
     window      {500, 1000, 2000, 4000}       microseconds per global iteration
-    waves       {0, 5}                        waves per SIMD of the compute kernel (0 = LDS bound)
+    waves       {0, 5}                        waves per SIMD of the compute kernel (0 = LDS bound;
+                                              workgroups of 256 only)
 Each trial times one batch solve after one warm-up solve.  Results are cached
 in a JSON file keyed by a mesh signature (sizes, bounding box, coordinate
 checksum) and the batch size.
@@ -81,7 +82,15 @@ def tune(mesh: Mesh, seeds, scales=None, space=None, cache=None, seed=0, rounds=
     trials = []
     seen = {}
 
+    def canonical(cfg):
+        # the register budget only selects a kernel variant for workgroups of
+        # 256; other workgroup sizes always run the 8-wave variant
+        if "waves" in cfg and cfg.get("block", DEFAULT["block"]) != 256:
+            return dict(cfg, waves=0)
+        return cfg
+
     def run(cfg):
+        cfg = canonical(cfg)
         k = json.dumps(cfg, sort_keys=True)
         if k not in seen:
             seen[k] = _measure(mesh, packs, cfg, seeds, scales, seed)
@@ -90,13 +99,13 @@ def tune(mesh: Mesh, seeds, scales=None, space=None, cache=None, seed=0, rounds=
                 log(f"[autotune] {cfg} -> {seen[k]:.1f} maps/s")
         return seen[k]
 
-    best = {k: (DEFAULT[k] if DEFAULT[k] in v else v[0]) for k, v in space.items()}
+    best = canonical({k: (DEFAULT[k] if DEFAULT[k] in v else v[0]) for k, v in space.items()})
     best_v = run(best)
     for _ in range(rounds):
         improved = False
         for k in [k for k in ("window", "max_nn", "block", "local_iters", "waves") if k in space]:
             for v in space[k]:
-                cand = dict(best, **{k: v})
+                cand = canonical(dict(best, **{k: v}))
                 r = run(cand)
                 if r > best_v:
                     best, best_v, improved = cand, r, True

@@ -367,3 +367,35 @@ def test_submesh_size_limits_are_refused():
     m = F.Mesh.gen_box(seed=1, **cases.scaled(cases.SLAB, 2))
     with pytest.raises(F.FimError, match="FIM_ELIMIT"):
         F.Pack(m, F.PART_RCB, max_nn=60000, seed=1)
+
+
+def test_autotune_search_logic_with_a_stand_in_timer(tmp_path, monkeypatch):
+    """Coordinate descent over the space with the timing replaced by a known
+    function of the configuration: the tuner must end at the known optimum,
+    report it as the best measured trial, skip register budgets that select
+    no kernel variant (workgroups other than 256), and serve the cache."""
+    import json
+    import fimpy.autotune as A
+    calls = []
+
+    def rate(cfg):  # optimum: 1536 copies, workgroups of 256 on 5 waves, 2 sweeps, 1 ms
+        return (10000.0 - abs(cfg["max_nn"] - 1536) - 100 * abs(cfg["local_iters"] - 2)
+                - abs(cfg["window"] - 1000.0) + (50 if cfg["block"] == 256 else 0)
+                + (20 if cfg["block"] == 256 and cfg["waves"] == 5 else 0))
+
+    def fake_measure(mesh, packs, cfg, seeds, scales, seed):
+        calls.append(dict(cfg))
+        return rate(cfg)
+
+    monkeypatch.setattr(A, "_measure", fake_measure)
+    monkeypatch.setattr(A, "signature", lambda mesh, n: f"stand-in/{n}")
+    cache = str(tmp_path / "tune.json")
+    best, trials = A.tune(None, [[0]] * 8, cache=cache, rounds=3)
+    assert best == {"max_nn": 1536, "block": 256, "local_iters": 2, "window": 1000.0, "waves": 5}
+    rates = {json.dumps(c, sort_keys=True): r for c, r in trials}
+    assert rates[json.dumps(best, sort_keys=True)] == max(rates.values())
+    assert all(c["waves"] == 0 for c in calls if c["block"] != 256)
+    assert len(rates) == len(calls)  # no configuration measured twice
+    n = len(calls)
+    again, _ = A.tune(None, [[0]] * 8, cache=cache)
+    assert again == best and len(calls) == n
