@@ -941,7 +941,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
                 ++vupd;
                 if (c < pp[3]) {
                   const uint32_t cbits = __float_as_uint(c);
-                  if (cbits < atomicMin(&phi_n[nt], cbits)) atomicOr(&wrt[nt >> 5], 1u << (nt & 31));
+                  // below the staged time: the minimum ends below it too, so the
+                  // written bit needs no returned value
+                  atomicMin(&phi_n[nt], cbits);
+                  atomicOr(&wrt[nt >> 5], 1u << (nt & 31));
                 }
               }
               qh += take;
