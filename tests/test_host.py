@@ -399,3 +399,57 @@ def test_autotune_search_logic_with_a_stand_in_timer(tmp_path, monkeypatch):
     n = len(calls)
     again, _ = A.tune(None, [[0]] * 8, cache=cache)
     assert again == best and len(calls) == n
+
+
+def test_stream_driver_bookkeeping_with_a_stand_in_solver():
+    """fimpy.stream.stream_solve against a stand-in solver whose solves take
+    a known number of iterations: every solve is reported exactly once, from
+    the instance it was seeded into, only after its iterations have run, and
+    a finished instance is recycled before it is seeded again."""
+    from fimpy.stream import stream_solve
+
+    class StandIn:
+        n_instances = 3
+
+        def __init__(self):
+            self.left = [0] * self.n_instances   # iterations still to run
+            self.seeded = [None] * self.n_instances
+            self.log = []
+            self.clock = 0
+
+        def reset(self):
+            self.log.append("reset")
+
+        def set_scaling(self, sc, inst):
+            self.log.append(("scale", inst))
+
+        def seed_batch(self, sets, t0, insts):
+            for s, i in zip(sets, insts):
+                assert self.left[i] == 0 and self.seeded[i] is None
+                self.seeded[i] = int(s[0])
+                self.left[i] = 1 + int(s[0]) % 7   # uneven solve lengths
+
+        def advance(self, iters, reset_stats=False):
+            self.clock += iters
+            self.left = [max(0, v - iters) for v in self.left]
+
+        def active(self):
+            return [1 if v > 0 else 0 for v in self.left]
+
+        def recycle(self, insts):
+            for i in insts:
+                assert self.left[i] == 0
+                self.seeded[i] = None
+
+    s = StandIn()
+    sets = [np.array([k]) for k in range(11)]
+    got = []
+
+    def on_map(k, inst):
+        assert s.seeded[inst] == k and s.left[inst] == 0
+        got.append(k)
+
+    iters = stream_solve(s, sets, scales=[[1.0] * 4] * len(sets), on_map=on_map, check_iters=2)
+    assert sorted(got) == list(range(len(sets)))
+    assert iters == s.clock > 0
+    assert s.log[0] == "reset" and sum(1 for e in s.log if e != "reset") == len(sets)
