@@ -643,6 +643,35 @@ int fim_nn_set_params(fim_nn *n, const float *params) {
   });
 }
 
+int fim_nn_adam_step(fim_nn *n, const float *grad) {
+  return guarded([&] {
+    if (!n || !grad) throw Error(FIM_EINVAL, "null argument");
+    HIPCHECK(hipSetDevice(n->dev));
+    std::vector<float> dev;
+    to_device_order(grad, dev);  // padding entries stay 0
+    HIPCHECK(hipMemcpyAsync(n->g, dev.data(), NPAD * sizeof(float), hipMemcpyHostToDevice, n->stream));
+    adam(n);
+    HIPCHECK(hipStreamSynchronize(n->stream));
+  });
+}
+
+int fim_nn_get_moments(fim_nn *n, float *m, float *v, int64_t *steps) {
+  return guarded([&] {
+    if (!n) throw Error(FIM_EINVAL, "null argument");
+    HIPCHECK(hipSetDevice(n->dev));
+    std::vector<float> dev(NPAD);
+    for (int k = 0; k < 2; ++k) {
+      float *dst = k == 0 ? m : v;
+      if (!dst) continue;
+      HIPCHECK(hipMemcpyAsync(dev.data(), k == 0 ? n->m : n->v, NPAD * sizeof(float), hipMemcpyDeviceToHost,
+                              n->stream));
+      HIPCHECK(hipStreamSynchronize(n->stream));
+      to_logical_order(dev, dst);
+    }
+    if (steps) *steps = n->t;
+  });
+}
+
 int fim_nn_features(fim_nn *n, const int64_t *vtx, const int64_t *sites, int32_t n_sites,
                     const float *scales, int64_t count, float *x) {
   return guarded([&] {

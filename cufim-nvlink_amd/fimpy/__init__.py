@@ -125,6 +125,8 @@ SIGNATURES = {
     "fim_nn_destroy": (None, [_vp]),
     "fim_nn_get_params": (C.c_int, [_vp, _pf]),
     "fim_nn_set_params": (C.c_int, [_vp, _pf]),
+    "fim_nn_adam_step": (C.c_int, [_vp, _pf]),
+    "fim_nn_get_moments": (C.c_int, [_vp, _pf, _pf, _pi64]),
     "fim_nn_features": (C.c_int, [_vp, _pi64, _pi64, C.c_int32, _pf, C.c_int64, _pf]),
     "fim_nn_add_maps": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, _pi64, _pf]),
     "fim_nn_dataset_size": (C.c_int, [_vp, _pi32]),
@@ -452,6 +454,20 @@ class Surrogate:
         p = np.ascontiguousarray(p, dtype=np.float32)
         assert p.shape == (NN_PARAMS,)
         _check(lib().fim_nn_set_params(self.h, _p(p, _pf)))
+
+    def adam_step(self, grad):
+        """One Adam step of the masters with a caller-given gradient."""
+        g = np.ascontiguousarray(grad, dtype=np.float32)
+        assert g.shape == (NN_PARAMS,)
+        _check(lib().fim_nn_adam_step(self.h, _p(g, _pf)))
+
+    def moments(self):
+        """(m, v, steps) of the optimiser, parameter order."""
+        m = np.zeros(NN_PARAMS, dtype=np.float32)
+        v = np.zeros(NN_PARAMS, dtype=np.float32)
+        t = C.c_int64()
+        _check(lib().fim_nn_get_moments(self.h, _p(m, _pf), _p(v, _pf), C.byref(t)))
+        return m, v, t.value
 
     def features(self, vtx, sites, scales):
         v = np.ascontiguousarray(vtx, dtype=np.int64).reshape(-1)

@@ -12,7 +12,10 @@ def stream_solve(solver, sets, scales=None, on_map=None, check_iters=6, max_iter
     """Solves len(sets) maps (pacing vertex arrays; scales: optional per-solve
     4-vectors of regional speed factors) through solver.n_instances slots.
     on_map(solve_index, inst) is called for every converged map before its
-    instance is reused.  Returns the number of global iterations run."""
+    instance is reused.  Returns the number of global iterations run, as the
+    device counts them (iterations that processed work; fim_advance runs
+    whole captured graphs, so one call may run more than check_iters);
+    max_iters caps that count."""
     n = solver.n_instances
     slot = [-1] * n
     solver.reset()
@@ -33,7 +36,7 @@ def stream_solve(solver, sets, scales=None, on_map=None, check_iters=6, max_iter
             raise RuntimeError("iteration cap reached before the stream was solved")
         solver.advance(check_iters, reset_stats=reset)
         reset = False
-        iters += check_iters
+        iters = int(solver.stats()["global_iters"])  # accumulated since the first advance
         act = solver.active()
         fin = [i for i in range(n) if slot[i] >= 0 and not act[i]]
         for i in fin:

@@ -326,6 +326,16 @@ void fim_nn_destroy(fim_nn *nn);
 /* FP32 master parameters in the order above (FIM_NN_PARAMS values) */
 int fim_nn_get_params(fim_nn *nn, float *params);
 int fim_nn_set_params(fim_nn *nn, const float *params);
+/* one Adam step of the FP32 masters with the given gradient (FIM_NN_PARAMS
+ * values, parameter order above): the update fim_nn_train applies after
+ * each backward pass -- m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
+ * w -= lr (m / (1-b1^t)) / (sqrt(v / (1-b2^t)) + eps), then the BF16 copies
+ * of the hidden-layer weights.  Lets a caller (or a test) drive the
+ * optimiser with a gradient of its own. */
+int fim_nn_adam_step(fim_nn *nn, const float *grad);
+/* Adam's first and second moments (FIM_NN_PARAMS values each, parameter
+ * order; either may be NULL) and the number of steps taken (may be NULL) */
+int fim_nn_get_moments(fim_nn *nn, float *m, float *v, int64_t *steps);
 /* inputs of one solve: sites (n <= 3 global vertex ids), scales[4] */
 int fim_nn_features(fim_nn *nn, const int64_t *vtx, const int64_t *sites, int32_t n_sites,
                     const float *scales, int64_t n, float *x);

@@ -156,14 +156,26 @@ double orc_tri_ref(double e12, double e13, double e23, double phi1,
 
 /* src/eikonal/cuda_fim.h:1302-1360, reproduced as written (including its
  * coefficient naming, which does not match the closed form it evaluates). */
+/* Planted mistakes, for tests only (orc_plant): the golden replay must
+ * reject each.  1: sqrt(sqt) where the formula has sqt; 2: the coefficients
+ * named as the distance term reads them (a = e34.e34, b = e13.e34,
+ * c = e13.e13, d = e13.e23, e = e23.e34, f = e23.e23); 3: no tet candidate
+ * at all (triangles and edges only). */
+static int g_plant = 0;
+void orc_plant(int which) { g_plant = which; }
+
 double orc_tet_ref(double e13, double e23, double e34, double e12, double e14,
                    double e24, double phi1, double phi2, double phi3) {
-  const double a = e13;
-  const double b = 0.5 * (e23 + e13 - e12);
-  const double c = -0.5 * (e34 + e13 - e14);
-  const double d = e23;
-  const double e = -0.5 * (e34 + e23 - e24);
-  const double f = e34;
+  if (g_plant == 3) return DMAX;
+  const double a0 = e13, b0 = 0.5 * (e23 + e13 - e12), c0 = -0.5 * (e34 + e13 - e14);
+  const double d0 = e23, e0 = -0.5 * (e34 + e23 - e24), f0 = e34;
+  const int rn = g_plant == 2;
+  const double a = rn ? f0 : a0;
+  const double b = rn ? c0 : b0;
+  const double c = rn ? a0 : c0;
+  const double d = rn ? b0 : d0;
+  const double e = e0;
+  const double f = rn ? d0 : f0;
   const double p13 = phi3 - phi1;
   const double p23 = phi3 - phi2;
 
@@ -175,6 +187,7 @@ double orc_tet_ref(double e13, double e23, double e34, double e12, double e14,
       2. * (e * c * d * d - b * d * d * d) * e +
       (e * e * c * c + (b * b - 2. * a * c) * d * d - 2. * (e * c * c - b * c * d) * e) * f;
   if (sqt < 0.) return DMAX;
+  const double sq = g_plant == 1 ? sqrt(sqt) : sqt;
 
   const double q1 = e * c * d * d - b * d * d * d + (e * c - b * d) * f * p13 * p13 -
                     2. * (e * c * d - b * d * d) * p13 * p23 +
@@ -183,8 +196,8 @@ double orc_tet_ref(double e13, double e23, double e34, double e12, double e14,
                      (d * d * f - c * f * f) * p13 * p13 -
                      2. * (d * d * d - c * d * f) * p13 * p23 +
                      (c * d * d - c * c * f) * p23 * p23);
-  const double m1 = (q1 + sqt * (d * p13 - c * p23)) / q2;
-  const double m2 = (q1 - sqt * (d * p13 - c * p23)) / q2;
+  const double m1 = (q1 + sq * (d * p13 - c * p23)) / q2;
+  const double m2 = (q1 - sq * (d * p13 - c * p23)) / q2;
   const double k1 = -((f * m1 + e) * p13 - (d * m1 + b) * p23) / (d * p13 - c * p23);
   const double k2 = -((f * m2 + e) * p13 - (d * m2 + b) * p23) / (d * p13 - c * p23);
 

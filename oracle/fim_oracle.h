@@ -60,6 +60,8 @@ void orc_geo_elem(const double *xyz, const int64_t con[4], const double M[6],
 double orc_tri_ref(double e12, double e13, double e23, double p1, double p2);
 
 /* reference tet candidate, formula as written (src/eikonal/cuda_fim.h:1302-1360) */
+/* test hook: plant a mistake in orc_tet_ref (0 = none; see fim_oracle.c) */
+void orc_plant(int which);
 double orc_tet_ref(double e13, double e23, double e34, double e12, double e14,
                    double e24, double p1, double p2, double p3);
 

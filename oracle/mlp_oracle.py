@@ -1,5 +1,6 @@
 """CPU restatement of the surrogate MLP (cufim-nvlink_amd/csrc/surrogate.hip):
-forward pass, squared-error loss and its gradient, in FP64 numpy.
+forward pass, squared-error loss and its gradient, and the Adam update, in
+FP64 numpy.
 
 TEST INFRASTRUCTURE ONLY: the parity oracle for the GPU network; only
 tests/ may import it.  The network has no counterpart in the reference.
@@ -9,6 +10,10 @@ same place (inputs, hidden-layer weights, pre-activations Z kept for the
 backward pass, activations H, backward dZ), so the two differ only by
 accumulation order (FP32 on the GPU).  With ``bf16=False`` the pass is
 exact FP64, and its gradient is pinned by central finite differences.
+
+Adam (``adam_step``) is Kingma & Ba's Algorithm 1 with bias-corrected
+moments; it is pinned by closed forms (first step, constant and alternating
+gradients) in tests/test_surrogate_oracle.py.
 
 Parameter order (333,569 values): W0[256 x 16], b0[256], then W_l[256 x 256],
 b_l[256] for l = 1..5, then w_out[256], b_out.
@@ -96,6 +101,17 @@ def loss_and_grad(p, x, t, bf16_emulation=True):
         flat += [gW.reshape(-1), gb]
     flat += [g_wo, np.array([g_bo])]
     return loss, np.concatenate(flat)
+
+
+def adam_step(p, m, v, g, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step (Kingma & Ba 2015, Algorithm 1): t is the step number
+    this update makes (1 for the first).  Returns (p, m, v) after it."""
+    p, m, v, g = (np.asarray(a, dtype=np.float64) for a in (p, m, v, g))
+    m = b1 * m + (1.0 - b1) * g
+    v = b2 * v + (1.0 - b2) * g * g
+    mh = m / (1.0 - b1 ** t)
+    vh = v / (1.0 - b2 ** t)
+    return p - lr * mh / (np.sqrt(vh) + eps), m, v
 
 
 def init_params(seed, scale=1.0):

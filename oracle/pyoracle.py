@@ -106,6 +106,11 @@ def tri_ref(e12, e13, e23, p1, p2):
     return lib().orc_tri_ref(e12, e13, e23, p1, p2)
 
 
+def plant(which):
+    """Test hook: plant mistake `which` in the reference tet formula (0 = none)."""
+    lib().orc_plant(int(which))
+
+
 def tet_ref(*a):
     return lib().orc_tet_ref(*a)
 

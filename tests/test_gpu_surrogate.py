@@ -76,6 +76,48 @@ def test_gradient_matches_oracle(small_slab):
     nn.close()
 
 
+def test_adam_steps_match_oracle(small_slab):
+    """k_adam against the oracle's Adam: three steps with known gradients
+    through the C ABI; FP32 masters and both moments agree with the FP64
+    update (tolerance: 4 FP32 ulps of the parameter plus 1e-5 of the step
+    size, far below the lr-sized differences a wrong bias correction or a
+    swapped beta gives).  The forward pass afterwards uses the updated BF16
+    copies of the hidden-layer weights: it matches the oracle forward with
+    the new parameters, and not the one with the old."""
+    _, m = small_slab
+    lr = 0.05
+    nn = F.Surrogate(m, seed=1, lr=lr)
+    p0 = M.init_params(9)
+    nn.set_params(p0)
+    rng = np.random.default_rng(4)
+    b1, b2, eps = float(np.float32(0.9)), float(np.float32(0.999)), float(np.float32(1e-8))
+    p, mo, vo = p0.astype(np.float64), np.zeros(M.NPARAMS), np.zeros(M.NPARAMS)
+    for t in (1, 2, 3):
+        g = (rng.normal(0, 1e-2, M.NPARAMS) * rng.choice([1e-3, 1.0, 1e2], M.NPARAMS)).astype(np.float32)
+        g[::97] = 0.0
+        if t == 2:
+            g[:5000] = -g[:5000]
+        nn.adam_step(g)
+        p, mo, vo = M.adam_step(p, mo, vo, g.astype(np.float64), t, lr, b1, b2, eps)
+    got = nn.params().astype(np.float64)
+    gm, gv, steps = nn.moments()
+    assert steps == 3
+    tol = 4 * np.spacing(np.abs(p).astype(np.float32)).astype(np.float64) + 1e-5 * lr
+    assert np.all(np.abs(got - p) <= tol), float(np.max(np.abs(got - p) - tol))
+    assert np.allclose(gm, mo, rtol=1e-5, atol=1e-12)
+    assert np.allclose(gv, vo, rtol=1e-5, atol=1e-18)
+    assert np.max(np.abs(got - p0)) > 0.1 * lr  # the steps moved the parameters
+    x = rng.uniform(-1, 1, (256, M.NIN)).astype(np.float32)
+    y = nn.predict(x)
+    new, _ = M.forward(got.astype(np.float32), x)
+    old, _ = M.forward(p0, x)
+    err_new = np.max(np.abs(y - new * TS))
+    err_old = np.max(np.abs(y - old * TS))
+    assert err_new <= 1e-2 * (np.ptp(new * TS) + 1e-3 * TS), err_new
+    assert err_old > 5 * err_new, (err_old, err_new)
+    nn.close()
+
+
 def test_features_layout(small_slab):
     cfg, m = small_slab
     nn = F.Surrogate(m, seed=1)

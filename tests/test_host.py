@@ -430,8 +430,14 @@ def test_stream_driver_bookkeeping_with_a_stand_in_solver():
                 self.left[i] = 1 + int(s[0]) % 7   # uneven solve lengths
 
         def advance(self, iters, reset_stats=False):
-            self.clock += iters
-            self.left = [max(0, v - iters) for v in self.left]
+            # like fim_advance: whole captured graphs of 6 iterations, counted
+            # by the device from the last reset_stats call
+            run = -(-iters // 6) * 6
+            self.clock = (0 if reset_stats else self.clock) + run
+            self.left = [max(0, v - run) for v in self.left]
+
+        def stats(self):
+            return {"global_iters": self.clock}
 
         def active(self):
             return [1 if v > 0 else 0 for v in self.left]
@@ -451,5 +457,27 @@ def test_stream_driver_bookkeeping_with_a_stand_in_solver():
 
     iters = stream_solve(s, sets, scales=[[1.0] * 4] * len(sets), on_map=on_map, check_iters=2)
     assert sorted(got) == list(range(len(sets)))
-    assert iters == s.clock > 0
+    assert iters == s.clock > 0 and iters % 6 == 0  # the iterations actually run
+    with pytest.raises(RuntimeError, match="iteration cap"):
+        stream_solve(StandIn(), sets, on_map=None, check_iters=2, max_iters=12)
     assert s.log[0] == "reset" and sum(1 for e in s.log if e != "reset") == len(sets)
+
+
+ADAPTER = os.path.join(ROOT, "oracle", "_ref", "adapter_run")
+
+
+@pytest.mark.skipif(not (os.path.isdir("/root/reference/src") and os.path.exists(ADAPTER)),
+                    reason="needs the reference sources and oracle/_ref/adapter_run")
+def test_integration_adapter_runs_from_the_reference_side():
+    """INTEGRATION.md's CufimSolver, compiled against the reference headers
+    and host code (`make -C oracle adapter`), takes the reference's mesh,
+    tags, options and plan through the C ABI: mesh and partition are built,
+    and without a GPU it stops with the device error at fim_create"""
+    U = cases.UT
+    r = subprocess.run([ADAPTER, f"{U}/ut_mesh.vtk", f"{U}/ut_mesh.tags",
+                        os.path.join(ROOT, "data", "default.cfg"), f"{U}/ut_mesh.plan.json",
+                        os.devnull], capture_output=True, text=True, timeout=60)
+    out = r.stdout + r.stderr
+    assert "num_stims: 1" in out  # the reference's plan parser ran
+    assert r.returncode == 0 or "hipSetDevice" in out, out
+

@@ -66,6 +66,35 @@ def test_golden_planted_mistakes_fail():
     assert not np.array_equal(st, gs)
 
 
+def test_golden_and_the_reference_tet_formula():
+    """What the golden pins of orc_tet_ref (cuda_fim.h:1302-1360): the
+    formula as written (sqt where sqrt(sqt) is dimensionally due) replays
+    it, and a plausible repair -- sqrt(sqt) -- does not.  Two other variants
+    replay it just as well: the coefficients named consistently with the
+    distance term, and no tet candidate at all.  So on ut_mesh the reference
+    tet formula never supplies a minimum, and the golden pins its rejection
+    behaviour, not its accepted values; the GPU compat kernel equals this
+    restatement bit for bit on every mesh tested."""
+    xyz, cells, tags = _ut()
+    m = O.Mesh(xyz, cells, atags=tags)
+    g = vtkio.read_vtk(f"{cases.UT}/cuda_fim_res.vtk")
+    gs = g["point_data"]["states"].astype(np.int32)
+    gp = g["point_data"]["phi"]
+
+    def replays(plant):
+        O.plant(plant)
+        try:
+            phi, st, _ = O.compat_solve(m, cases.ut_plan_stimuli(), 150e3, 3)
+        finally:
+            O.plant(0)
+        ours = np.where(phi < 1e50, phi, -1.0)
+        return bool(np.array_equal(st, gs) and np.max(np.abs(ours - gp)) <= 5.0001e-7)
+
+    assert replays(0)
+    assert not replays(1)          # sqrt(sqt): the golden rejects it
+    assert replays(2) and replays(3)  # not distinguished by the golden
+
+
 # ------------------------------------------------- reference host code ----
 @need_ref
 def test_connectivity_matches_reference_host_code():

@@ -56,3 +56,52 @@ def test_bf16_emulation_stays_close_to_exact():
     assert abs(l1 - l0) <= 0.05 * l0
     cos = g1 @ g0 / np.linalg.norm(g1) / np.linalg.norm(g0)
     assert cos > 0.99
+
+
+def test_oracle_adam_closed_forms_and_planted_mistakes():
+    """Adam pinned by what its definition fixes: the first step moves every
+    parameter by lr * g / (|g| + eps) (the bias corrections cancel the
+    (1 - beta) factors); a constant gradient keeps m_hat = g, v_hat = g^2 at
+    every step; after g, -g the corrected moments are -(1 - b1) / (1 + b1) g
+    and g^2.  A dropped bias correction, swapped betas or eps inside the
+    square root each fail one of these."""
+    rng = np.random.default_rng(3)
+    n = 1000
+    p0 = rng.normal(0, 0.1, n)
+    g = rng.normal(0, 1e-2, n) * rng.choice([1e-3, 1.0, 1e3], n)
+    z = np.zeros(n)
+    lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+
+    def close(a, b):
+        return np.allclose(a, b, rtol=1e-12, atol=1e-15)
+
+    def checks(step):
+        p1, m1, v1 = step(p0, z, z, g, 1)
+        ok = close(p1, p0 - lr * g / (np.abs(g) + eps))
+        p, m, v = p0, z, z
+        for t in range(1, 6):
+            q, m, v = step(p, m, v, g, t)
+            ok &= close(q, p - lr * g / (np.abs(g) + eps))
+            p = q
+        p2, m2, v2 = step(p1, m1, v1, -g, 2)
+        mh = -(1 - b1) / (1 + b1) * g
+        ok &= close(p2, p1 - lr * mh / (np.abs(g) + eps))
+        return bool(ok)
+
+    assert checks(lambda p, m, v, gg, t: M.adam_step(p, m, v, gg, t, lr, b1, b2, eps))
+
+    def no_correction(p, m, v, gg, t):
+        m = b1 * m + (1 - b1) * gg
+        v = b2 * v + (1 - b2) * gg * gg
+        return p - lr * m / (np.sqrt(v) + eps), m, v
+
+    def swapped(p, m, v, gg, t):
+        return M.adam_step(p, m, v, gg, t, lr, b2, b1, eps)
+
+    def eps_inside(p, m, v, gg, t):
+        m = b1 * m + (1 - b1) * gg
+        v = b2 * v + (1 - b2) * gg * gg
+        return p - lr * (m / (1 - b1 ** t)) / np.sqrt(v / (1 - b2 ** t) + eps), m, v
+
+    for bad in (no_correction, swapped, eps_inside):
+        assert not checks(bad), bad.__name__
