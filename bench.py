@@ -6,26 +6,29 @@ Workload (one GPU): the anisotropic slab -- 160 x 160 x 13 hexes x 6 tets =
 -60..+60 degrees through the thickness, v_fibre 0.6 and v_cross 0.24 m/s --
 with five pacing vertices per map on the z = 0 face.  One step = one batch of
 B maps (different pacing sets) solved to the eps = 1e-6 fixed point by the
-batched HIP solver, mesh resident in HBM; reset and seeding are inside the
-timed region.  The maps of the first instances (one per host thread, at most
-16) are checked against the FP64 fixed points of the CPU oracle (max |dT| <=
-0.1 ms over them is required for a valid number); those oracle runs, one map
-per host thread, are the CPU baseline.
-Plain single-solve GPU FIM (B = 1) is timed beside it.
+batched HIP solver on the library's default path (iterations replayed from
+a captured hipGraph), mesh resident in HBM; reset and seeding are inside the
+timed region.  `value` is that rate.
 
-With --nn the surrogate warm start (coordinate MLP on BF16 MFMA) is measured
-in the same run on the same slab with quadrant speed factors: 4,096 training maps
-from the batched solver, Adam training, the upward shift delta calibrated on
-64 validation solves, then 512 held-out solves warm-started (prediction +
-delta, all vertices active, support check as judge, cold re-solve as
-fallback) against the same 512 solved cold.  Its numbers are reported in
-`nn_warm_start`; the headline value is the batched solver alone, which is
-the faster of the two on this workload (the warm start is 3x slower).
-
-With --compat-steps N the reference's time-window algorithm (compat mode,
-FP64) runs N windows of one map on the same slab, on the GPU and then on one
-host thread with the oracle, and the two maps are compared bit for bit
-(`compat` in the line).
+Legs of the same run, in this order:
+  accounting   two batches of the same maps with every compute launch timed
+               by HIP events on the solver's stream (profile mode, the same
+               kernel sequence the graph replays); these are the first
+               k_conv_band dispatches of the process, and the roofline and
+               the per-launch figures come from them;
+  headline     W warm-up and K timed batches, graph replay;
+  plain        plain single-solve GPU FIM (one map per solve, graph replay,
+               its own best settings) -- the target's denominator;
+  compat       the reference's time-window algorithm (compat mode, FP64) on
+               one map: N fim_step windows, timed like the reference (step
+               loop only);
+  nn (--nn)    the surrogate-scheduled batched solve (see nn_leg);
+  CPU          the same compat map on all host cores (the oracle's OpenMP
+               restatement of the reference algorithm, step loop timed): the
+               CPU baseline, compared with the GPU compat map bit for bit;
+               then the FP64 oracle's fixed points of the first maps of the
+               batch, one per host thread, which validate the number (max |dT|
+               <= 0.1 ms) and are reported as a second CPU figure.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
@@ -44,11 +47,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cufim-nvlink_amd"))
 
 import fimpy as F  # noqa: E402
-from fimpy import warmstart as WS  # noqa: E402
 from fimpy import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
-METRIC = "maps/s @max|ΔT|≤0.1ms, 2M-tet slab: batched+NN warm-start vs plain FIM"
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every
+# 4 cycles at 2.4 GHz
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+METRIC = ("maps/s @max|ΔT|≤0.1ms, 2M-tet slab: batched+NN warm-start vs plain FIM "
+          "[value: batched FIM alone; the surrogate leg is nn_warm_start]")
+NN_PARAMS = 333569
 
 
 def log(*a):
@@ -65,28 +73,10 @@ def solve_batch(s, sets):
     return s.run(max_iters=1_000_000)
 
 
-def traffic_from_profiles(args):
-    """HBM bytes per compute launch from the committed PMC summary, when this
-    run's configuration is the one that was profiled (else None)."""
-    p = os.path.join(ROOT, "profiles", "conv_band_slab_pmc.md")
-    if not os.path.exists(p):
-        return None
-    text = open(p).read()
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
-    c = re.search(r"traffic_config:\s*(.*)", text)
-    mine = (f"batch={args.batch} max_nn={args.max_nn} block={args.block} unit={args.unit} "
-            f"local_iters={args.local_iters} "
-            f"window={float(args.window)} staged={args.staged} nn={args.nn}")
-    src = re.search(r"traffic_source:\s*([0-9a-f]+)", text)
-    if not m or not c or c.group(1).strip() != mine or not src or src.group(1) != kernel_source_digest():
-        return None
-    return float(m.group(1))
-
-
 def kernel_source_digest():
     """sha256 (first 16 hex digits) of the kernel sources without // comments
-    and blank lines; a PMC summary applies only to the code it was measured
-    on."""
+    and blank lines; a counter summary applies only to the code it was
+    measured on."""
     import hashlib
     h = hashlib.sha256()
     for f in ("solver.hip", "local_solve.hpp"):
@@ -97,7 +87,33 @@ def kernel_source_digest():
     return h.hexdigest()[:16]
 
 
-NN_PARAMS = 333569
+def config_key(args):
+    return (f"batch={args.batch} max_nn={args.max_nn} block={args.block} unit={args.unit} "
+            f"local_iters={args.local_iters} window={float(args.window)}")
+
+
+def profile_figures(args, name):
+    """key: value figures of a committed counter summary, when it was
+    measured on this configuration and this kernel source (else None)."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    text = open(p).read()
+    src = re.search(r"^source:\s*([0-9a-f]+)", text, re.M)
+    cfg = re.search(r"^config:\s*(.*)$", text, re.M)
+    if not src or src.group(1) != kernel_source_digest() or not cfg or cfg.group(1).strip() != config_key(args):
+        return None
+    return {k: float(v) for k, v in re.findall(r"^([a-z_]+):\s*([-0-9.eE+]+)\s*$", text, re.M)}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def compat_leg(args, mesh, sites):
@@ -126,16 +142,20 @@ def compat_leg(args, mesh, sites):
     e2s, _ = pack.partition()
     rep = {"max_nn": args.compat_max_nn, "submeshes": pack.info["nsub"], "dt_us": args.compat_dt,
            "steps": args.compat_steps, "gpu_ms_per_step": 1e3 * best / args.compat_steps,
-           "finite": int(np.sum(phi < 1e50))}
-    log(f"[bench] compat: {args.compat_steps} steps of {args.compat_dt:.0f} us, "
-        f"{rep['gpu_ms_per_step']:.3f} ms per step on the GPU")
+           "gpu_maps_per_s": 1.0 / best, "reached": int(np.sum(phi < 1e50))}
+    log(f"[bench] compat: {args.compat_steps} windows of {args.compat_dt:.0f} us, "
+        f"{rep['gpu_ms_per_step']:.3f} ms per window on the GPU")
     return rep, (phi, st, e2s, stims, fw)
 
 
-def nn_leg(args, account, plain_s):
-    """Surrogate warm start on the slab with quadrant speed factors; returns
-    the report dict."""
-    nb = args.nn_solve_batch  # instances per batched solve of this leg
+def nn_leg(args, plain_s):
+    """Surrogate warm start on the slab with quadrant speed factors: training
+    maps from the batched solver, Adam training on MFMA, the upward shift
+    delta calibrated on validation solves, then held-out solves warm-started
+    (prediction + delta, all vertices active, support check as judge, cold
+    re-solve as fallback) against the same solves run cold."""
+    from fimpy import warmstart as WS
+    nb = args.nn_solve_batch
     cfg = W.SLAB
     mesh = W.box(cfg, args.seed)
     mesh.set_regions(W.quadrant_regions(mesh))
@@ -143,45 +163,41 @@ def nn_leg(args, account, plain_s):
     train = WS.draw_solves(cfg, args.nn_maps, 1001 + args.seed)
     val = WS.draw_solves(cfg, 64, 2002 + args.seed)
     test = WS.draw_solves(cfg, args.nn_test, 3003 + args.seed)
-    kw = dict(batch=nb, window=args.window, profile=True, on_stats=account,
-              local_iters=args.local_iters, block=args.block)
+    kw = dict(batch=nb, window=args.window, local_iters=args.local_iters, block=args.block)
     nn = F.Surrogate(mesh, seed=args.seed, max_maps=args.nn_maps, max_batch=args.nn_batch)
     t_data = WS.build_dataset(nn, pack, train, unit=args.unit, **kw)
     log(f"[bench] surrogate dataset: {args.nn_maps} maps in {t_data:.2f}s")
     t0 = time.perf_counter()
     done, loss = 0, float("nan")
     while done < args.nn_steps:
-        k = min(500, args.nn_steps - done)
+        k = min(1000, args.nn_steps - done)
         loss = nn.train(k, batch=args.nn_batch, seed=args.seed * 7919 + done)
         done += k
         log(f"[bench] surrogate training {done}/{args.nn_steps} steps, loss {loss:.3e}")
     t_train = time.perf_counter() - t0
-    log(f"[bench] surrogate trained: {args.nn_steps} steps of {args.nn_batch} in {t_train:.2f}s, loss {loss:.3e}")
     errs = WS.surrogate_errors(nn, pack, val, **kw)
     under = max(e[0] for e in errs)
     delta = 1.25 * max(under, 0.0)
-    # held-out solves: cold batched (timed), surrogate error on each map
-    s = F.Solver(pack, n_instances=nb, window=args.window, unit=args.unit, profile=True,
-                 local_iters=args.local_iters, block=args.block)
-    t_cold, cold_it, cold_ev, sur = 0.0, 0, 0, []
+    s = F.Solver(pack, n_instances=nb, window=args.window, unit=args.unit, local_iters=args.local_iters,
+                 block=args.block)
+    t_cold, cold_it, cold_ev, cold_act, sur = 0.0, 0, 0, 0, []
     for i in range(0, len(test), nb):
         chunk = test[i:i + nb]
         full = chunk + [chunk[0]] * (nb - len(chunk))
         t0 = time.perf_counter()
         st = WS.cold_batch(s, full)
         t_cold += time.perf_counter() - t0
-        account(st)
         cold_it += st["global_iters"]
         cold_ev += st["elem_updates"]
+        cold_act += st["activations"]
         for j, (sites, sc) in enumerate(chunk):
             ref, _ = s.download(inst=j)
             d = np.abs(nn.predict_map(sites, sc) - ref)
             sur.append((float(d.mean()), float(d.max()), float(ref.max())))
-        if i == 0:
-            map0 = s.download(inst=0)[0]
     s.close()
     rep = WS.warm_solve_all(nn, pack, test, delta, unit=args.unit, all_active=True, **kw)
     t_warm = rep["infer_s"] + rep["fim_s"] + rep["judge_s"] + rep["fallback_s"]
+    nn.close()
     out = {
         "method": "prediction + delta as the initial field, all vertices active, batched FIM to eps 1e-6; "
                   "support check as judge, cold re-solve of rejected maps",
@@ -194,8 +210,8 @@ def nn_leg(args, account, plain_s):
         "warm_over_plain": (rep["maps"] / t_warm) * plain_s,
         "seconds": {"inference": rep["infer_s"], "fim": rep["fim_s"], "judge": rep["judge_s"],
                     "fallback": rep["fallback_s"]},
-        "global_iters": {"cold": cold_it, "warm": rep["global_iters"],
-                         "saved": cold_it - rep["global_iters"]},
+        "global_iters": {"cold": cold_it, "warm": rep["global_iters"]},
+        "activations": {"cold": cold_act, "warm": rep["activations"]},
         "elem_evals": {"cold": cold_ev, "warm": rep["elem_updates"]},
         "delta_us": delta,
         "fallbacks": rep["fallbacks"],
@@ -206,24 +222,10 @@ def nn_leg(args, account, plain_s):
                      "batch": args.nn_batch, "train_s": t_train, "final_loss": loss,
                      "tflops": 6.0 * NN_PARAMS * args.nn_batch * args.nn_steps / t_train / 1e12},
         "inference_tflops": 2.0 * NN_PARAMS * ((mesh.nn + 63) // 64 * 64) * rep["maps"] / rep["infer_s"] / 1e12,
-        "mfma_peak_tflops": 2500.0,
+        "mfma_peak_tflops": MFMA_BF16_PEAK_TFLOPS,
     }
-    if not args.no_check:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as O  # accuracy check only
-        xyz, e2n, _ = mesh.arrays()
-        fib, vel = mesh.material()
-        sites, sc = test[0]
-        reg = W.quadrant_regions(mesh)
-        om = O.Mesh(xyz, e2n, fib=fib, vel=vel * sc.astype(np.float64)[reg][:, None])
-        ref, _ = O.fim_solve(om, sites, variant=O.LOCAL_EXACT)
-        pred = nn.predict_map(sites, sc)
-        out["surrogate_vs_oracle_us"] = {"mean": float(np.mean(np.abs(pred - ref))),
-                                         "max": float(np.max(np.abs(pred - ref)))}
-        out["fim_vs_oracle_max_us"] = float(np.max(np.abs(map0 - ref)))
-    nn.close()
     log(f"[bench] warm start: {out['maps_per_s']:.1f} maps/s vs cold {out['cold_maps_per_s']:.1f} "
-        f"(iterations {cold_it} -> {rep['global_iters']}, {rep['fallbacks']} fallbacks)")
+        f"({rep['fallbacks']} fallbacks)")
     return out
 
 
@@ -241,14 +243,9 @@ def main():
     ap.add_argument("--unit", type=int, default=2,
                     help="queued instances of one submesh run back to back by one workgroup")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--plain-steps", type=int, default=3)
-    ap.add_argument("--graph-steps", type=int, default=0,
-                    help="also time this many steps with graph-captured iterations")
-    ap.add_argument("--stream-steps", type=int, default=0,
-                    help="also time this many batches of maps streamed through the instances "
-                         "(continuous batching: a converged instance takes the next map)")
-    ap.add_argument("--staged", action="store_true",
-                    help="stage whole submeshes in LDS (default: element data from L2)")
+    ap.add_argument("--account-steps", type=int, default=2,
+                    help="batches solved with every compute launch timed (roofline)")
+    ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--window", type=float, default=2000.0,
                     help="time window per global iteration in us (0 = plain FIM order)")
     # plain single-solve GPU FIM: its own best setting, found by a sweep of
@@ -258,15 +255,12 @@ def main():
     ap.add_argument("--plain-window", type=float, default=2000.0)
     ap.add_argument("--plain-local-iters", type=int, default=3)
     ap.add_argument("--plain-block", type=int, default=512)
-    ap.add_argument("--plain-waves", type=int, default=0,
-                    help="waves per SIMD of the plain solve's compute kernel (0 = LDS bound)")
-    ap.add_argument("--compat-steps", type=int, default=0,
-                    help="also time the reference time-window algorithm (compat mode, FP64) for this "
-                         "many steps on the slab, GPU and host oracle (about a minute)")
+    ap.add_argument("--compat-steps", type=int, default=150,
+                    help="windows of the reference time-window algorithm (compat mode, FP64) on one "
+                         "map, GPU and all host cores (0 = skip); 150 x 2 ms covers the slab's maps")
     ap.add_argument("--compat-dt", type=float, default=2000.0)
     ap.add_argument("--compat-max-nn", type=int, default=128)
-    ap.add_argument("--nn", action="store_true",
-                    help="also measure the surrogate warm start (adds about a minute)")
+    ap.add_argument("--nn", action="store_true", help="also measure the surrogate leg")
     ap.add_argument("--nn-maps", type=int, default=4096)
     ap.add_argument("--nn-steps", type=int, default=4000)
     ap.add_argument("--nn-batch", type=int, default=65536)
@@ -275,7 +269,9 @@ def main():
     ap.add_argument("--nn-test", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0,
-                    help="host threads of the CPU baseline, one map each (0 = min(16, cores))")
+                    help="host threads of the CPU legs (0 = all cores this process may use)")
+    ap.add_argument("--check-maps", type=int, default=16,
+                    help="maps of the batch checked against the FP64 oracle (one per host thread)")
     ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
 
@@ -293,156 +289,153 @@ def main():
     log(f"[bench] mesh {mesh.nn} vertices {mesh.ne} tets, {info['nsub']} submeshes, "
         f"setup {time.perf_counter() - t0:.2f}s")
     sets = pacing_sets(cfg, args.batch, args.seed)
+    host_cores = len(os.sched_getaffinity(0))
+    n_threads = args.cpu_threads if args.cpu_threads > 0 else host_cores
 
-    # Every solve below launches its iterations individually with a HIP
-    # event pair around each compute-kernel launch on the solver's stream
-    # (profile mode: the same kernel sequence the graph replays).  The
-    # roofline uses the average over all those launches, which is also what
-    # `rocprofv3 --kernel-trace --stats` reports for this command.
-    kern = dict(alg_bytes=0, compute_ms=0.0, compute_launches=0)
-
-    def account(st):
+    # ---- accounting: the first k_conv_band dispatches of the process, each
+    # bracketed by HIP events on the solver's stream -------------------------
+    acc = F.Solver(pack, n_instances=args.batch, block=args.block, profile=True, unit=args.unit,
+                   local_iters=args.local_iters, window=args.window)
+    kern = dict(alg_bytes=0, compute_ms=0.0, compute_launches=0, elem_updates=0, activations=0,
+                vertex_updates=0, global_iters=0, local_iters=0)
+    for i in range(args.account_steps):
+        solve_batch(acc, sets)
+        st = acc.stats()
         for k in kern:
             kern[k] += st[k]
+        log(f"[bench] accounting batch {i + 1}/{args.account_steps}: {st['compute_launches']} launches, "
+            f"{st['compute_ms']:.1f} ms of compute kernel")
+    acc_map0 = acc.download(inst=0)[0]
+    acc.close()
 
-    # ---- batched solver, timed -----------------------------------------
-    s = F.Solver(pack, n_instances=args.batch, block=args.block, profile=True, unit=args.unit,
-                 local_iters=args.local_iters,
-                 window=args.window, staged=args.staged)
+    # ---- headline: graph replay (the library default) -----------------------
+    s = F.Solver(pack, n_instances=args.batch, block=args.block, unit=args.unit,
+                 local_iters=args.local_iters, window=args.window)
     for i in range(args.warmup):
         solve_batch(s, sets)
-        account(s.stats())
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
     tot = dict(elem_updates=0, global_iters=0, activations=0, local_iters=0, vertex_updates=0)
     t0 = time.perf_counter()
     for i in range(args.steps):
         solve_batch(s, sets)
         st = s.stats()
-        account(st)
         for k in tot:
             tot[k] += st[k]
         log(f"[bench] step {i + 1}/{args.steps}")
     elapsed = time.perf_counter() - t0
     log(f"[bench] timed {args.steps} steps of {args.batch} maps in {elapsed:.3f}s")
-    n_cpu = args.cpu_threads if args.cpu_threads > 0 else min(16, len(os.sched_getaffinity(0)))
-    n_cpu = max(1, min(n_cpu, args.batch))
-    gpu_maps = [s.download(inst=i)[0] for i in range(n_cpu)]
-    phi0 = gpu_maps[0]
+    n_check = max(1, min(args.check_maps, args.batch))
+    gpu_maps = [s.download(inst=i)[0] for i in range(n_check)]
     res0 = s.residual(inst=0)
     s.close()
+    assert np.array_equal(gpu_maps[0], acc_map0), "graph replay differs from direct launches"
     ms_per_step = 1e3 * elapsed / args.steps
     maps_per_s = args.batch * args.steps / elapsed
 
-    # ---- optionally: continuous batching of stream_steps x batch further
-    # maps through the same number of instances (graph-captured iterations)
-    stream_maps_per_s = None
-    if args.stream_steps > 0:
-        from fimpy.stream import stream_solve
-        g = F.Solver(pack, n_instances=args.batch, block=args.block, window=args.window, unit=args.unit,
-                     local_iters=args.local_iters, staged=args.staged)
-        ssets = pacing_sets(cfg, args.batch * (args.stream_steps + 1), args.seed + 1)
-        stream_solve(g, ssets[:args.batch])  # warm-up
-        t0 = time.perf_counter()
-        stream_solve(g, ssets[args.batch:])
-        stream_maps_per_s = args.batch * args.stream_steps / (time.perf_counter() - t0)
-        g.close()
-        log(f"[bench] streamed {args.batch * args.stream_steps} maps: {stream_maps_per_s:.1f} maps/s")
-
-    # ---- optionally: the same batch with graph-captured iterations (the
-    # library default; its launches cannot carry events, so it is off here
-    # to keep every compute launch of this command timed)
-    graph_maps_per_s = None
-    if args.graph_steps > 0:
-        g = F.Solver(pack, n_instances=args.batch, block=args.block, window=args.window, unit=args.unit,
-                     local_iters=args.local_iters,
-                     staged=args.staged)
-        solve_batch(g, sets)
-        t0 = time.perf_counter()
-        for i in range(args.graph_steps):
-            solve_batch(g, sets)
-        graph_maps_per_s = args.batch * args.graph_steps / (time.perf_counter() - t0)
-        phi0g, _ = g.download(inst=0)
-        g.close()
-        assert np.array_equal(phi0, phi0g), "graph replay differs from direct launches"
-
-    # ---- plain single-solve GPU FIM (the denominator of the target) -----
+    # ---- plain single-solve GPU FIM (the denominator of the target) --------
     ppack = pack if args.plain_max_nn == args.max_nn else \
         F.Pack(mesh, F.PART_RCB, max_nn=args.plain_max_nn, seed=args.seed)
-    p1 = F.Solver(ppack, n_instances=1, block=args.plain_block, profile=True, window=args.plain_window,
-                  local_iters=args.plain_local_iters, waves=args.plain_waves,
-                  staged=args.staged)
+    p1 = F.Solver(ppack, n_instances=1, block=args.plain_block, window=args.plain_window,
+                  local_iters=args.plain_local_iters)
     solve_batch(p1, sets[:1])
-    account(p1.stats())
     t0 = time.perf_counter()
     for i in range(args.plain_steps):
         solve_batch(p1, sets[i % len(sets):i % len(sets) + 1])
-        account(p1.stats())
     plain_s = (time.perf_counter() - t0) / args.plain_steps
     p1.close()
-    log(f"[bench] plain single solve {1e3 * plain_s:.2f} ms")
+    log(f"[bench] plain single solve {1e3 * plain_s:.2f} ms (graph replay)")
 
-    # ---- surrogate warm start ---------------------------------------------
-    nnrep = nn_leg(args, account, plain_s) if args.nn else None
+    # ---- surrogate leg -------------------------------------------------------
+    nnrep = nn_leg(args, plain_s) if args.nn else None
 
-    # ---- reference time-window algorithm (compat mode) ---------------------
+    # ---- reference time-window algorithm (compat mode) ----------------------
     compat, compat_replay = (None, None)
     if args.compat_steps > 0:
         compat, compat_replay = compat_leg(args, mesh, sets[0])
 
-    # ---- accuracy and CPU baseline (FP64 oracle, one map per host thread) --
-    max_dt = None
-    cpu = None
+    # ---- CPU legs: the oracle is the checker and the CPU baseline only ------
+    max_dt, cpu, cpu_fp64 = None, None, None
     if not (args.no_check and args.no_cpu_baseline):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as O  # accuracy check and CPU baseline only
+        import pyoracle as O  # accuracy check and CPU baselines only
         from concurrent.futures import ThreadPoolExecutor
         xyz, e2n, _ = mesh.arrays()
         fib, vel = mesh.material()
-        om = O.Mesh(xyz, e2n, fib=fib, vel=vel)
-
-        def oracle_map(i):  # the C oracle releases the GIL
-            t_start = time.perf_counter()
-            ref_i, _ = O.fim_solve(om, sets[i], variant=O.LOCAL_EXACT)
-            return ref_i, time.perf_counter() - t_start
-
-        log(f"[bench] CPU oracle: {n_cpu} maps on {n_cpu} host threads")
-        t0 = time.perf_counter()
-        with ThreadPoolExecutor(n_cpu) as ex:
-            res = list(ex.map(oracle_map, range(n_cpu)))
-        wall = time.perf_counter() - t0
-        max_dt = 0.0
-        for (ref, _), gmap in zip(res, gpu_maps):
-            fin = ref < 1e50
-            if not np.array_equal(fin, gmap < 1e50):
-                max_dt = float("inf")
-            else:
-                max_dt = max(max_dt, float(np.max(np.abs(gmap[fin] - ref[fin]))))
-        times = [dt for _, dt in res]
-        log(f"[bench] oracle {n_cpu} maps in {wall:.2f}s (per map {min(times):.1f}-{max(times):.1f}s), "
-            f"max|dT| over them = {max_dt:.3f} us")
         if compat_replay is not None and not args.no_cpu_baseline:
-            # the same compat solve on one host thread, same partition
             cphi, cst, e2s, stims, fw = compat_replay
             omc = O.Mesh(xyz, e2n, fib=fib, vel=vel, atags=e2s.astype(np.int32))
             t0 = time.perf_counter()
-            ophi, ost, _ = O.compat_solve(omc, stims, args.compat_dt, args.compat_steps, front_width=fw)
-            tc = time.perf_counter() - t0
-            compat["cpu_ms_per_step"] = 1e3 * tc / args.compat_steps
-            compat["cpu_cores"] = 1
-            compat["gpu_over_cpu"] = compat["cpu_ms_per_step"] / compat["gpu_ms_per_step"]
+            ophi, ost, sec = O.compat_solve_par(omc, stims, args.compat_dt, args.compat_steps, n_threads,
+                                                front_width=fw)
+            wall = time.perf_counter() - t0
             compat["bit_identical"] = bool(np.array_equal(cphi, ophi) and np.array_equal(cst, ost))
-            log(f"[bench] compat on one host thread: {compat['cpu_ms_per_step']:.2f} ms per step "
-                f"(setup included), bit-identical {compat['bit_identical']}")
-        cpu = {"value": n_cpu / float(np.mean(times)), "unit": "maps/s", "cores": n_cpu, "kind": "port",
-               "sample": f"{n_cpu} maps of the same slab and pacing sets (instances 0..{n_cpu - 1}), one "
-                         f"per host thread, FP64 label-correcting fixed point of the same local update; "
-                         f"value = threads / mean time per map ({min(times):.1f}-{max(times):.1f} s, "
-                         f"wall {wall:.1f} s)"}
+            compat["cpu_ms_per_step"] = 1e3 * sec[1] / args.compat_steps
+            compat["gpu_over_cpu"] = compat["cpu_ms_per_step"] / compat["gpu_ms_per_step"]
+            log(f"[bench] compat on {n_threads} host threads: step loop {sec[1]:.2f}s, setup {sec[0]:.2f}s, "
+                f"bit-identical to the GPU {compat['bit_identical']}")
+            cpu = {"value": 1.0 / sec[1], "unit": "maps/s", "cores": n_threads, "kind": "port",
+                   "sample": f"one map of the slab (pacing set 0), the reference's time-window algorithm "
+                             f"(cuda_fim.cu:1752-2334) as the oracle's FP64 restatement on {n_threads} "
+                             f"OpenMP threads, one submesh per thread at a time ({compat['submeshes']} "
+                             f"submeshes of <= {args.compat_max_nn} copies, own partition): "
+                             f"{args.compat_steps} windows of {args.compat_dt:.0f} us; value = 1 / step-loop "
+                             f"seconds ({sec[1]:.2f} s), timed as the reference times its solve",
+                   "end_to_end_maps_per_s": 1.0 / wall, "setup_s": float(sec[0]),
+                   "nproc": host_cores, "cpu_model": cpu_model()}
+        if not args.no_check:
+            om = O.Mesh(xyz, e2n, fib=fib, vel=vel)
+
+            def oracle_map(i):  # the C oracle releases the GIL
+                t_start = time.perf_counter()
+                ref_i, _ = O.fim_solve(om, sets[i], variant=O.LOCAL_EXACT)
+                return ref_i, time.perf_counter() - t_start
+
+            nth = min(n_check, n_threads)
+            log(f"[bench] FP64 oracle: {n_check} maps on {nth} host threads")
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(nth) as ex:
+                res = list(ex.map(oracle_map, range(n_check)))
+            wall = time.perf_counter() - t0
+            max_dt = 0.0
+            for (ref, _), gmap in zip(res, gpu_maps):
+                fin = ref < 1e50
+                if not np.array_equal(fin, gmap < 1e50):
+                    max_dt = float("inf")
+                else:
+                    max_dt = max(max_dt, float(np.max(np.abs(gmap[fin] - ref[fin]))))
+            times = [dt for _, dt in res]
+            log(f"[bench] oracle {n_check} maps in {wall:.2f}s (per map {min(times):.1f}-{max(times):.1f}s), "
+                f"max|dT| over them = {max_dt:.3f} us")
+            cpu_fp64 = {"value": n_check / wall, "unit": "maps/s", "threads": nth,
+                        "sample": f"maps 0..{n_check - 1} of the batch, FP64 label-correcting fixed point "
+                                  f"(exact local solver), one map per host thread; {wall:.1f} s wall"}
+            if compat_replay is not None:
+                ref0 = res[0][0]
+                fin = (ref0 < 1e50) & (compat_replay[0] < 1e50)
+                compat["max_abs_dT_vs_fp64_exact_us"] = float(np.max(np.abs(compat_replay[0][fin] - ref0[fin])))
     valid = max_dt is not None and max_dt <= 100.0
 
-    bytes_per_launch = kern["alg_bytes"] / max(kern["compute_launches"], 1)
-    avg_launch_s = kern["compute_ms"] / 1e3 / max(kern["compute_launches"], 1)
+    # ---- roofline of k_conv_band from the accounting launches ---------------
+    nl = max(kern["compute_launches"], 1)
+    bytes_per_launch = kern["alg_bytes"] / nl
+    avg_launch_s = kern["compute_ms"] / 1e3 / nl
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    acc_maps = args.batch * args.account_steps
+    pmc = profile_figures(args, "conv_band_slab_counters.md")
+    valu = None
+    traffic = None
+    if pmc:
+        traffic = pmc.get("traffic_bytes_per_launch")
+        ipl = pmc.get("valu_insts_per_launch")
+        if ipl:
+            valu = {"insts_per_launch": ipl,
+                    "issue_frac": ipl / avg_launch_s / VALU_ISSUE_PEAK if avg_launch_s > 0 else None,
+                    "insts_per_candidate": ipl / (kern["vertex_updates"] / nl) if kern["vertex_updates"] else None,
+                    "lane_slots_per_candidate": 64 * ipl / (kern["vertex_updates"] / nl) if kern["vertex_updates"] else None,
+                    "lds_conflict_cycles_per_lds_inst": pmc.get("lds_conflicts_per_lds_inst"),
+                    "wait_frac": pmc.get("wait_frac"),
+                    "peak_insts_per_s": VALU_ISSUE_PEAK}
+    one_pass = 32.0 * mesh.ne + 10.0 * mesh.nn
     out = {
         "metric": METRIC,
         "value": maps_per_s if (valid or args.no_check) else 0.0,
@@ -459,8 +452,8 @@ def main():
         "config": {
             "workload": "anisotropic slab 160x160x13 hexes x 6 = 1,996,800 tets / 362,894 vertices, "
                         "fibres -60..+60 deg, v_f 0.6 / v_c 0.24 m/s, 5 pacing sites per map; "
-                        f"batch of {args.batch} maps per step solved to eps 1e-6 "
-                        "(batched FIM; the surrogate warm start, slower here, is measured with --nn)",
+                        f"batch of {args.batch} maps per step solved to eps 1e-6 by the batched FIM "
+                        "(iterations replayed from a captured hipGraph)",
             "maps_per_step": args.batch,
             "partition": f"recursive bisection, <= {args.max_nn} vertex copies per submesh, "
                          f"{info['nsub']} submeshes",
@@ -469,37 +462,47 @@ def main():
             "unit": args.unit,
             "local_iters": args.local_iters,
             "window_us": args.window,
-            "staged": args.staged,
         },
         "max_abs_dT_us": max_dt,
         "tolerance_us": 100.0,
         "valid": valid,
         "residual_inst0": res0,
-        "graph_maps_per_s": graph_maps_per_s,
-        "stream_maps_per_s": stream_maps_per_s,
         "plain_maps_per_s": 1.0 / plain_s,
         "plain_config": {"max_nn": args.plain_max_nn, "window_us": args.plain_window,
                          "local_iters": args.plain_local_iters, "workgroup": args.plain_block,
-                         "waves": args.plain_waves},
+                         "timing": "graph replay, one map per solve"},
         "batched_over_plain": maps_per_s * plain_s,
         "solves_per_s": maps_per_s,
         "elem_evals_per_s": tot["elem_updates"] / elapsed,
         "vertex_updates_per_s": tot["vertex_updates"] / elapsed,
         "global_iters_per_step": tot["global_iters"] / args.steps,
         "activations_per_step": tot["activations"] / args.steps,
+        "sweep_overhead": {
+            "activations_per_submesh_per_map": kern["activations"] / acc_maps / info["nsub"],
+            "elem_evals_per_elem_per_map": kern["elem_updates"] / acc_maps / mesh.ne,
+            "candidates_per_vertex_per_map": kern["vertex_updates"] / acc_maps / mesh.nn,
+            "sweeps_per_activation": kern["local_iters"] / max(kern["activations"], 1),
+            "alg_bytes_per_map_over_one_pass": kern["alg_bytes"] / acc_maps / one_pass,
+            "one_pass_bytes_per_map": one_pass,
+        },
         "roofline": {
-            "bound": "hbm",
-            "kernel": "k_conv_compute" if args.staged else "k_conv_band",
+            "bound": "valu" if valu else "hbm",
+            "kernel": "k_conv_band",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(args),
+            "traffic": traffic,
             "alg_bytes_per_launch": bytes_per_launch,
+            "bytes_per_unit": "32 per distinct element evaluated + 10 per vertex copy + 12 per interface "
+                              "copy of each activation",
             "avg_launch_us": 1e6 * avg_launch_s,
             "launches": kern["compute_launches"],
+            "launches_are": "the first k_conv_band dispatches of the process (accounting batches)",
+            "valu": valu,
         },
         "cpu_baseline": cpu,
+        "cpu_fp64_oracle": cpu_fp64,
         "nn_warm_start": nnrep,
         "compat": compat,
     }

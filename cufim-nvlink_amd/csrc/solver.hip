@@ -268,7 +268,8 @@ struct ConvArgs {
   const uint4 *adj;         // 8 incident local element ids (u16) per chunk, 0xffff padded
   uint32_t nsub, ncopy, ninst;
   float eps;
-  float window;          // per-iteration time window, 0 = off
+  const float *winst;    // n_inst: time window added per global iteration of
+                         // each instance (us), 0 = off (plain FIM order)
   int32_t max_local;
   uint32_t unit, maxch;  // instances per work unit, units per submesh bucket
   uint32_t n_if, wps;    // interface entries; ifm words per submesh
@@ -340,7 +341,8 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
   uint32_t inst, s;
   while (cursor.next(inst, s)) {
     const size_t entry = (size_t)inst * a.nsub + s;
-    const float W = a.window > 0.0f ? a.window * (float)(iter_now - a.start[inst] + 1u) : F_INF;
+    const float wi = a.winst[inst];
+    const float W = wi > 0.0f ? wi * (float)(iter_now - a.start[inst] + 1u) : F_INF;
     const SubInfo si = a.sub[s];
     const size_t cb = (size_t)inst * a.ncopy + si.coff;
     const uint32_t nn = si.nn, ne = si.ne;
@@ -586,7 +588,7 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
       __syncthreads();
       used = misc[3];
     }
-    abytes += 48ull * used + 10ull * nn + 12ull * si.ni;
+    abytes += 32ull * used + 10ull * nn + 12ull * si.ni;
     // write back decreased copies and push the lowered interface copies.
     // Deferred vertices (and sources left by the sweep cap) stay dirty and
     // the submesh is queued for the next iteration.
@@ -729,7 +731,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
   uint32_t inst, s;
   while (cursor.next(inst, s)) {
     const size_t entry = (size_t)inst * a.nsub + s;
-    const float W = a.window > 0.0f ? a.window * (float)(iter_now - a.start[inst] + 1u) : F_INF;
+    const float wi = a.winst[inst];
+    const float W = wi > 0.0f ? wi * (float)(iter_now - a.start[inst] + 1u) : F_INF;
     const SubInfo si = a.sub[s];
     const size_t cb = (size_t)inst * a.ncopy + si.coff;
     const uint32_t nn = si.nn, ne = si.ne;
@@ -1058,7 +1061,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
     sweeps += it;
     acts += 1;
     // algorithmic bytes: copies (phi 4+4, flag 1+1), interface entries
-    // (4+4+4) and the distinct elements evaluated (48 B record)
+    // (4+4+4) and the distinct elements evaluated (32 B: 24 B of FP32
+    // geometry + 8 B of node ids; the 16 B of node heights in the record
+    // are pruning data of this kernel, not algorithmic bytes)
     uint32_t cnt = 0;
     for (uint32_t k = tid; k < new32; k += BS) cnt += __popc(eseen[k]);
     cnt = wave_sum_small<9>(cnt);  // at most 8 words of 32 per thread
@@ -1105,7 +1110,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
     __syncthreads();
     if (tid == 0) {
       a.pcnt[entry] = misc[6];
-      abytes += 48ull * misc[3] + 10ull * nn + 12ull * si.ni;
+      abytes += 32ull * misc[3] + 10ull * nn + 12ull * si.ni;
       const float wk = __uint_as_float(misc[5]);
       a.wake[entry] = wk;
       if (wk < F_INF) requeue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
@@ -1627,6 +1632,7 @@ struct fim_ctx {
   uint8_t *d_dirty = nullptr;
   uint32_t *d_act = nullptr, *d_cnt = nullptr;
   uint32_t *d_start = nullptr;  // n_inst: iteration each instance was (re)seeded at
+  float *d_winst = nullptr;     // n_inst: time window per iteration (us), 0 = off
   uint32_t *d_flag = nullptr;   // n_inst: scratch for instance queries / lists
   uint2 *d_prec = nullptr;  // push records: n_inst x (interface entries)
   uint32_t *d_pcnt = nullptr;
@@ -1858,6 +1864,25 @@ void ctx_init_field(fim_ctx *c, int32_t inst, bool all_active) {
   HIPCHECK(hipGetLastError());
 }
 
+// support check of one instance with `best` (nn words of device memory) as
+// the per-vertex candidate minimum
+double support_into(fim_ctx *c, int32_t inst, uint32_t *best) {
+  k_fill<uint32_t><<<blocks_for(c->nn, 256), 256, 0, c->stream>>>(best, c->nn, 0x7f800000u);
+  k_conv_support<<<(int)std::min<uint32_t>(c->nsub, 4096u), 256, 0, c->stream>>>(
+      c->d_sub, c->nsub, c->d_e2n, c->d_geo32, c->d_ereg, c->d_rscale, c->d_phi, c->d_c2g,
+      (size_t)inst * c->ncopy, (uint32_t)inst, best);
+  k_conv_gather<<<blocks_for(c->nn, 256), 256, 0, c->stream>>>(c->d_phi, c->d_g2c_dsp, c->d_g2c, c->nn,
+                                                               (size_t)inst * c->ncopy, c->d_tmpf);
+  k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_tmpu, 1, 0u);
+  k_support_check<<<blocks_for(c->nn, 256), 256, 0, c->stream>>>(c->d_tmpf, best, c->nn, c->d_tmpu);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(c->h_pin, c->d_tmpu, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  float r;
+  std::memcpy(&r, c->h_pin, 4);
+  return r;
+}
+
 }  // namespace fim
 
 extern "C" {
@@ -2014,6 +2039,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
           c->d_prec = dalloc<uint2>((size_t)c->ninst * p.if_l.size(), ow);
           c->d_pcnt = dalloc<uint32_t>(nss, ow);
           c->d_start = dalloc<uint32_t>(c->ninst, ow);
+          c->d_winst = dupload(std::vector<float>(c->ninst, (float)std::max(0.0, o->window)), ow);
           c->d_flag = dalloc<uint32_t>(c->ninst, ow);
           HIPCHECK(hipMemset(c->d_start, 0, sizeof(uint32_t) * c->ninst));
           // per submesh: interface-copy bits of each 32 local copies and the
@@ -2078,7 +2104,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
                          c->d_pcnt, WorkList{}, WorkList{}, c->d_stats, c->d_cnt + 48, c->d_start,
                          c->d_n2e_dsp, c->d_n2e_le, c->d_erec, c->d_adj_ptr, c->d_adj,
                          c->nsub, c->ncopy, c->ninst, (float)(o->eps_rel > 0 ? o->eps_rel : 1e-6),
-                         (float)o->window, o->local_iters, c->unit, c->maxch, (uint32_t)p.if_l.size(),
+                         c->d_winst, o->local_iters, c->unit, c->maxch, (uint32_t)p.if_l.size(),
                          c->wps, elcap, o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc,
                          o_wrt, o_ifc};
         c->sa = SyncArgs{c->d_sub, c->d_if_fo, c->d_if_os, c->d_prec, c->d_pcnt,
@@ -2232,6 +2258,19 @@ int fim_set_scaling(fim_ctx *c, int32_t inst, const float scale[4]) {
     }
     HIPCHECK(hipSetDevice(c->dev));
     HIPCHECK(hipMemcpyAsync(c->d_rscale + 4 * (size_t)inst, m, sizeof(m), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int fim_set_window(fim_ctx *c, int32_t inst, double window) {
+  return guarded([&] {
+    if (!c) throw Error(FIM_EINVAL, "null context");
+    if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "time windows need converge mode");
+    check_inst(c, inst);
+    if (!(window >= 0.0) || !(window < 1e30)) throw Error(FIM_EINVAL, "window must be >= 0 and finite");
+    const float w = (float)window;
+    HIPCHECK(hipSetDevice(c->dev));
+    HIPCHECK(hipMemcpyAsync(c->d_winst + inst, &w, sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
   });
 }
@@ -2481,21 +2520,7 @@ int fim_support(fim_ctx *c, int32_t inst, double *worst) {
     uint32_t *best = nullptr;
     HIPCHECK(hipMalloc(&best, (size_t)c->nn * sizeof(uint32_t)));
     try {
-      k_fill<uint32_t><<<blocks_for(c->nn, 256), 256, 0, c->stream>>>(best, c->nn, 0x7f800000u);
-      k_conv_support<<<(int)std::min<uint32_t>(c->nsub, 4096u), 256, 0, c->stream>>>(
-          c->d_sub, c->nsub, c->d_e2n, c->d_geo32, c->d_ereg, c->d_rscale, c->d_phi, c->d_c2g,
-          (size_t)inst * c->ncopy, (uint32_t)inst, best);
-      k_conv_gather<<<blocks_for(c->nn, 256), 256, 0, c->stream>>>(c->d_phi, c->d_g2c_dsp, c->d_g2c, c->nn,
-                                                                   (size_t)inst * c->ncopy, c->d_tmpf);
-      k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_tmpu, 1, 0u);
-      k_support_check<<<blocks_for(c->nn, 256), 256, 0, c->stream>>>(c->d_tmpf, best, c->nn, c->d_tmpu);
-      HIPCHECK(hipGetLastError());
-      uint32_t bits = 0;
-      HIPCHECK(hipMemcpyAsync(&bits, c->d_tmpu, 4, hipMemcpyDeviceToHost, c->stream));
-      HIPCHECK(hipStreamSynchronize(c->stream));
-      float r;
-      std::memcpy(&r, &bits, 4);
-      *worst = r;
+      *worst = support_into(c, inst, best);
     } catch (...) {
       (void)hipFree(best);
       throw;

@@ -139,6 +139,110 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, si
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same product on 128 x 128 workgroup tiles: four waves of 64 x 64, each
+// wave 2 x 2 MFMA accumulators (32 x 32 each), so every A and B fragment read
+// from LDS feeds two MFMAs.  K advances 32 at a time through two LDS buffers:
+// the next step's global loads are issued into registers before the current
+// step's MFMAs and stored into the other buffer after them, one barrier per
+// step.  R, Cc multiples of 128; the K span a multiple of 32.
+// ---------------------------------------------------------------------------
+template <bool AK, bool BK, class Epi>
+__global__ __launch_bounds__(256) void k_gemm128(const uint16_t *__restrict__ A, size_t sar, size_t sak,
+                                                 const uint16_t *__restrict__ B, size_t sbk, size_t sbc,
+                                                 uint32_t kspan, Epi epi) {
+  constexpr int LD = 40;  // LDS row pitch in BF16 values (80 B)
+  __shared__ __align__(16) uint16_t As[2][128][LD];  // [buffer][row][k]
+  __shared__ __align__(16) uint16_t Bs[2][128][LD];  // [buffer][col][k]
+  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t wr = w & 1u, wc = w >> 1;
+  const size_t r0 = (size_t)blockIdx.y * 128, c0 = (size_t)blockIdx.x * 128;
+  const size_t kb = (size_t)blockIdx.z * kspan;
+  const uint32_t nk = kspan / 32;
+  u16x8 ra[2], rb[2];
+  auto gload = [&](size_t k0) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (AK) {
+        const uint32_t r = (tid >> 2) + 64u * j, kq = (tid & 3u) * 8u;
+        ra[j] = *reinterpret_cast<const u16x8 *>(A + (r0 + r) * sar + k0 + kq);
+      } else {
+        const uint32_t k = (tid >> 4) + 16u * j, rq = (tid & 15u) * 8u;
+        ra[j] = *reinterpret_cast<const u16x8 *>(A + (k0 + k) * sak + r0 + rq);
+      }
+      if (BK) {
+        const uint32_t c = (tid >> 2) + 64u * j, kq = (tid & 3u) * 8u;
+        rb[j] = *reinterpret_cast<const u16x8 *>(B + (c0 + c) * sbc + k0 + kq);
+      } else {
+        const uint32_t k = (tid >> 4) + 16u * j, cq = (tid & 15u) * 8u;
+        rb[j] = *reinterpret_cast<const u16x8 *>(B + (k0 + k) * sbk + c0 + cq);
+      }
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (AK) {
+        const uint32_t r = (tid >> 2) + 64u * j, kq = (tid & 3u) * 8u;
+        *reinterpret_cast<u16x8 *>(&As[buf][r][kq]) = ra[j];
+      } else {
+        const uint32_t k = (tid >> 4) + 16u * j, rq = (tid & 15u) * 8u;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) As[buf][rq + i][k] = ra[j][i];
+      }
+      if (BK) {
+        const uint32_t c = (tid >> 2) + 64u * j, kq = (tid & 3u) * 8u;
+        *reinterpret_cast<u16x8 *>(&Bs[buf][c][kq]) = rb[j];
+      } else {
+        const uint32_t k = (tid >> 4) + 16u * j, cq = (tid & 15u) * 8u;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Bs[buf][cq + i][k] = rb[j][i];
+      }
+    }
+  };
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+  gload(kb);
+  lstore(0);
+  __syncthreads();
+  for (uint32_t st = 0; st < nk; ++st) {
+    const int buf = st & 1;
+    if (st + 1 < nk) gload(kb + 32 * (size_t)(st + 1));
+#pragma unroll
+    for (uint32_t kk = 0; kk < 32; kk += 16) {
+      u16x8 a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        a[i] = *reinterpret_cast<const u16x8 *>(&As[buf][wr * 64 + i * 32 + (lane & 31u)][kk + (lane >> 5) * 8]);
+        b[i] = *reinterpret_cast<const u16x8 *>(&Bs[buf][wc * 64 + i * 32 + (lane & 31u)][kk + (lane >> 5) * 8]);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a[i]),
+                                                             __builtin_bit_cast(bf16x8, b[j]), acc[i][j], 0, 0, 0);
+    }
+    if (st + 1 < nk) lstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const size_t row = r0 + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const size_t col = c0 + wc * 64 + j * 32 + (lane & 31u);
+        epi(row, col, acc[i][j][r]);
+      }
+}
+
 // hidden layer forward: Z = acc + b (BF16, kept for the backward pass when Z
 // is given), H = SiLU(Z)
 struct EpiFwd {
@@ -205,6 +309,28 @@ __global__ void k_map_rows(const float *coords, const float *mp, uint32_t nn, ui
                            uint16_t *X0) {
   for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < rows; m += gridDim.x * blockDim.x)
     write_features(X0 + (size_t)m * K0, coords + 3 * (size_t)(m < nn ? m : 0), mp);
+}
+
+// rows of several solves over one vertex sample: row r is solve r / m at
+// sample vertex r % m (padding rows repeat the last real row)
+__global__ void k_sample_rows(const float *coords, const float *mp, const uint32_t *sample, uint32_t m,
+                              uint32_t nreal, uint32_t rows, uint16_t *X0) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
+    const uint32_t q = r < nreal ? r : nreal - 1;
+    const uint32_t j = q / m, v = sample[q - j * m];
+    write_features(X0 + (size_t)r * K0, coords + 3 * (size_t)v, mp + MAPP * (size_t)j);
+  }
+}
+
+// per solve: the largest of its m predictions (one wave per solve)
+__global__ __launch_bounds__(256) void k_solve_max(const float *t, uint32_t m, uint32_t nsolve, float *out) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t j = blockIdx.x * 4 + wave; j < nsolve; j += gridDim.x * 4) {
+    float mx = -__builtin_huge_valf();
+    for (uint32_t i = lane; i < m; i += 64) mx = fmaxf(mx, t[(size_t)j * m + i]);
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    if (lane == 0) out[j] = mx;
+  }
 }
 
 // given inputs (n x 16 floats) -> padded BF16 rows
@@ -403,12 +529,16 @@ void ensure_rows(fim_nn *n, uint32_t rows) {
 
 // forward pass over `rows` rows of X0 (store_z keeps Z for backward)
 void forward(fim_nn *n, uint32_t rows, bool store_z) {
-  const dim3 g(HID / 64, rows / 64, 1);
+  const bool big = rows % 128 == 0;
+  const dim3 g = big ? dim3(HID / 128, rows / 128, 1) : dim3(HID / 64, rows / 64, 1);
   for (int l = 0; l < NL; ++l) {
     const uint16_t *in = l == 0 ? n->X0 : n->H[l - 1];
     const size_t K = l == 0 ? K0 : HID;
     EpiFwd e{n->w + off_b(l), store_z ? n->Z[l] : nullptr, n->H[l]};
-    k_gemm<true, true, EpiFwd><<<g, 256, 0, n->stream>>>(in, K, 1, n->wb + offb_w(l), 1, K, (uint32_t)K, e);
+    if (big)
+      k_gemm128<true, true, EpiFwd><<<g, 256, 0, n->stream>>>(in, K, 1, n->wb + offb_w(l), 1, K, (uint32_t)K, e);
+    else
+      k_gemm<true, true, EpiFwd><<<g, 256, 0, n->stream>>>(in, K, 1, n->wb + offb_w(l), 1, K, (uint32_t)K, e);
   }
   HIPCHECK(hipGetLastError());
 }
@@ -431,14 +561,23 @@ void backward(fim_nn *n, uint32_t rows) {
     // dW_l[n][k] = sum_m dZ[m][n] hin[m][k]: A(r=n, k=m) = dZ (contiguous in r),
     // B(k=m, c=k) = hin (contiguous in c)
     EpiPart ep{n->parts, K};
-    k_gemm<false, false, EpiPart><<<dim3(K / 64, HID / 64, S), 256, 0, n->stream>>>(dz, 1, HID, hin, K, 1, span, ep);
+    if (K % 128 == 0)
+      k_gemm128<false, false, EpiPart><<<dim3(K / 128, HID / 128, S), 256, 0, n->stream>>>(dz, 1, HID, hin, K, 1,
+                                                                                           span, ep);
+    else
+      k_gemm<false, false, EpiPart><<<dim3(K / 64, HID / 64, S), 256, 0, n->stream>>>(dz, 1, HID, hin, K, 1, span,
+                                                                                       ep);
     k_reduce_parts<<<grid_for((size_t)HID * K), 256, 0, n->stream>>>(n->parts, S, (size_t)HID * K, n->g + off_w(l));
     k_colsum<<<std::min<uint32_t>(rows, 1024), 256, 0, n->stream>>>(dz, rows, n->g + off_b(l));
     if (l > 0) {
       // dZ_{l-1}[m][k] = sum_n dZ[m][n] W_l[n][k] * SiLU'(Z_{l-1})
       EpiBwd eb{n->Z[l - 1], dzp};
-      k_gemm<true, false, EpiBwd><<<dim3(HID / 64, rows / 64, 1), 256, 0, n->stream>>>(
-          dz, HID, 1, n->wb + offb_w(l), HID, 1, HID, eb);
+      if (rows % 128 == 0)
+        k_gemm128<true, false, EpiBwd><<<dim3(HID / 128, rows / 128, 1), 256, 0, n->stream>>>(
+            dz, HID, 1, n->wb + offb_w(l), HID, 1, HID, eb);
+      else
+        k_gemm<true, false, EpiBwd><<<dim3(HID / 64, rows / 64, 1), 256, 0, n->stream>>>(
+            dz, HID, 1, n->wb + offb_w(l), HID, 1, HID, eb);
       std::swap(dz, dzp);
     }
   }
@@ -512,7 +651,7 @@ int guarded(F &&f) {
 
 // prediction for all vertices of one solve into dst (device, nn floats)
 void predict_map_device(fim_nn *n, const std::vector<float> &mp, float delta, float *dst) {
-  const uint32_t rows = (n->nn + 63u) & ~63u;
+  const uint32_t rows = (n->nn + 127u) & ~127u;  // 128-row tiles
   ensure_rows(n, rows);
   HIPCHECK(hipMemcpyAsync(n->mparams + (size_t)n->max_maps * MAPP, mp.data(), MAPP * sizeof(float),
                           hipMemcpyHostToDevice, n->stream));
@@ -562,8 +701,8 @@ int fim_nn_create(const fim_mesh *mesh, const fim_nn_opts *o, fim_nn **out) {
       n->max_maps = o->max_maps > 0 ? o->max_maps : 4096;
       const uint32_t mb = o->max_batch > 0 ? (uint32_t)o->max_batch : 65536u;
       if (mb % 64) throw Error(FIM_EINVAL, "max_batch must be a multiple of 64");
-      n->cap = std::max<uint32_t>(mb, (n->nn + 63u) & ~63u);
-      n->nsplit = 32;
+      n->cap = std::max<uint32_t>(mb, (n->nn + 127u) & ~127u);
+      n->nsplit = 128;  // batch splits of the weight gradients: 4 x 128 tiles of 128 x 128
       // parameters: He-uniform hidden layers, small output layer, zero biases
       std::vector<float> w(NPAD, 0.0f);
       uint64_t st = o->seed ^ 0x5DEECE66Dull;
@@ -775,7 +914,8 @@ int fim_nn_predict(fim_nn *n, const float *x, int64_t count, float *t) {
   return guarded([&] {
     if (!n || (!x && count > 0) || (!t && count > 0) || count < 0) throw Error(FIM_EINVAL, "null argument");
     if (count == 0) return;
-    const uint32_t rows = (uint32_t)((count + 63) & ~int64_t(63));
+    const uint32_t r128 = (uint32_t)((count + 127) & ~int64_t(127));
+    const uint32_t rows = r128 <= n->cap ? r128 : (uint32_t)((count + 63) & ~int64_t(63));
     ensure_rows(n, rows);
     HIPCHECK(hipSetDevice(n->dev));
     float *xd = nullptr;
@@ -794,6 +934,64 @@ int fim_nn_predict(fim_nn *n, const float *x, int64_t count, float *t) {
       throw;
     }
     HIPCHECK(hipFree(xd));
+  });
+}
+
+int fim_nn_predict_max(fim_nn *n, int32_t count, const int64_t *sites, const float *scales,
+                       const int64_t *sample, int32_t m, float *out) {
+  return guarded([&] {
+    if (!n || count < 0 || (count > 0 && (!sites || !scales || !out)) || m <= 0 || !sample)
+      throw Error(FIM_EINVAL, "bad arguments");
+    if (count == 0) return;
+    if ((uint32_t)m > n->cap) throw Error(FIM_ELIMIT, "vertex sample larger than the row capacity");
+    std::vector<uint32_t> sv(m);
+    for (int32_t i = 0; i < m; ++i) {
+      if (sample[i] < 0 || sample[i] >= (int64_t)n->nn) throw Error(FIM_EINVAL, "sample vertex out of range");
+      sv[i] = (uint32_t)sample[i];
+    }
+    std::vector<float> mp((size_t)count * MAPP);
+    for (int32_t j = 0; j < count; ++j) {
+      int32_t ns = 0;
+      while (ns < 3 && sites[3 * j + ns] >= 0) ++ns;
+      const std::vector<float> p = map_params(n, sites + 3 * j, ns, scales + 4 * j);
+      std::copy(p.begin(), p.end(), mp.begin() + (size_t)j * MAPP);
+    }
+    HIPCHECK(hipSetDevice(n->dev));
+    // solves per forward pass: as many whole samples as the rows hold
+    const uint32_t per = std::max<uint32_t>(1u, n->cap / (uint32_t)m);
+    uint32_t *sd = nullptr;
+    float *md = nullptr, *od = nullptr;
+    try {
+      HIPCHECK(hipMalloc(&sd, sv.size() * sizeof(uint32_t)));
+      HIPCHECK(hipMalloc(&md, mp.size() * sizeof(float)));
+      HIPCHECK(hipMalloc(&od, (size_t)count * sizeof(float)));
+      HIPCHECK(hipMemcpyAsync(sd, sv.data(), sv.size() * sizeof(uint32_t), hipMemcpyHostToDevice, n->stream));
+      HIPCHECK(hipMemcpyAsync(md, mp.data(), mp.size() * sizeof(float), hipMemcpyHostToDevice, n->stream));
+      for (uint32_t j0 = 0; j0 < (uint32_t)count; j0 += per) {
+        const uint32_t nj = std::min<uint32_t>(per, (uint32_t)count - j0);
+        const uint32_t nreal = nj * (uint32_t)m;
+        uint32_t rows = (nreal + 127u) & ~127u;
+        if (rows > n->cap) rows = (nreal + 63u) & ~63u;
+        k_sample_rows<<<grid_for(rows), 256, 0, n->stream>>>(n->coords, md + (size_t)j0 * MAPP, sd, (uint32_t)m,
+                                                             nreal, rows, n->X0);
+        forward(n, rows, false);
+        k_head_infer<<<std::min<uint32_t>(rows / 4, 4096), 256, 0, n->stream>>>(
+            n->H[NL - 1], n->w + OFF_WO, n->w + OFF_BO, nreal, t_scale(n), 0.0f, n->out);
+        k_solve_max<<<std::min<uint32_t>((nj + 3) / 4, 4096), 256, 0, n->stream>>>(n->out, (uint32_t)m, nj,
+                                                                                  od + j0);
+        HIPCHECK(hipGetLastError());
+      }
+      HIPCHECK(hipMemcpyAsync(out, od, (size_t)count * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+      HIPCHECK(hipStreamSynchronize(n->stream));
+    } catch (...) {
+      (void)hipFree(sd);
+      (void)hipFree(md);
+      (void)hipFree(od);
+      throw;
+    }
+    HIPCHECK(hipFree(sd));
+    HIPCHECK(hipFree(md));
+    HIPCHECK(hipFree(od));
   });
 }
 

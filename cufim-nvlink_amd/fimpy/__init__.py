@@ -116,6 +116,7 @@ SIGNATURES = {
     "fim_recycle": (C.c_int, [_vp, C.c_int32, _pi32]),
     "fim_set_scaling": (C.c_int, [_vp, C.c_int32, _pf]),
     "fim_init_field": (C.c_int, [_vp, C.c_int32, _pf]),
+    "fim_set_window": (C.c_int, [_vp, C.c_int32, C.c_double]),
     "fim_step": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _pi32]),
     "fim_run": (C.c_int, [_vp, C.c_int32, _pi32]),
     "fim_download": (C.c_int, [_vp, C.c_int32, _pd, _pi32]),
@@ -134,6 +135,7 @@ SIGNATURES = {
     "fim_nn_train": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_uint64, _pd]),
     "fim_nn_predict": (C.c_int, [_vp, _pf, C.c_int64, _pf]),
     "fim_nn_predict_map": (C.c_int, [_vp, _pi64, C.c_int32, _pf, _pf]),
+    "fim_nn_predict_max": (C.c_int, [_vp, C.c_int32, _pi64, _pf, _pi64, C.c_int32, _pf]),
     "fim_nn_warm_start": (C.c_int, [_vp, _vp, C.c_int32, _pi64, C.c_int32, _pf, C.c_double, C.c_int32]),
     "fim_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
     "fim_destroy": (None, [_vp]),
@@ -322,6 +324,10 @@ class Solver:
         s = np.ascontiguousarray(scale, dtype=np.float32)
         assert s.shape == (4,)
         _check(lib().fim_set_scaling(self.h, inst, _p(s, _pf)))
+
+    def set_window(self, window, inst=0):
+        """Time window (us per global iteration) of one instance; 0 = off."""
+        _check(lib().fim_set_window(self.h, inst, float(window)))
 
     def init_field(self, phi0, inst=0):
         p = np.ascontiguousarray(phi0, dtype=np.float32)
@@ -515,6 +521,19 @@ class Surrogate:
         t = np.zeros(self.nn, dtype=np.float32)
         _check(lib().fim_nn_predict_map(self.h, _p(s, _pi64), ns, _p(sc, _pf), _p(t, _pf)))
         return t
+
+    def predict_max(self, solves, sample):
+        """Latest predicted activation time of each (sites, scales) solve over
+        the vertex sample."""
+        n = len(solves)
+        s = np.stack([_sites3(x[0])[0] for x in solves]).astype(np.int64) if n else np.zeros((0, 3), np.int64)
+        sc = np.ascontiguousarray(np.stack([np.asarray(x[1], np.float32) for x in solves]) if n
+                                  else np.zeros((0, 4), np.float32))
+        v = np.ascontiguousarray(sample, dtype=np.int64).reshape(-1)
+        out = np.zeros(n, dtype=np.float32)
+        _check(lib().fim_nn_predict_max(self.h, n, _p(s, _pi64), _p(sc, _pf), _p(v, _pi64), v.shape[0],
+                                        _p(out, _pf)))
+        return out
 
     def warm_start(self, solver, inst, sites, scales, delta, all_active=True):
         s, ns = _sites3(sites)

@@ -224,8 +224,9 @@ typedef struct {
   int64_t elem_updates;     /* element evaluations                          */
   double last_ms;           /* device time of the last fim_run / fim_step   */
   int64_t alg_bytes;        /* algorithmic HBM bytes of the activations:
-                               48 per distinct element evaluated + 10 per
-                               vertex copy + 12 per interface copy of each
+                               32 per distinct element evaluated (FP32
+                               geometry + u16 node ids) + 10 per vertex
+                               copy + 12 per interface copy of each
                                activated submesh                            */
   int64_t compute_launches; /* compute-kernel launches timed (profile)      */
   double compute_ms;        /* summed compute-kernel time (profile)         */
@@ -247,6 +248,12 @@ int fim_seed_batch(fim_ctx *c, int32_t n_inst, const int32_t *inst, const int64_
                    const int64_t *vtx, double t0);
 /* converge mode: per-instance, per-region scaling of both speeds (4 regions) */
 int fim_set_scaling(fim_ctx *c, int32_t inst, const float scale[4]);
+/* converge mode: the time window of one instance (us added per global
+ * iteration, counted from the iteration it was seeded at; 0 = plain FIM
+ * order), in place of fim_opts.window.  The fixed point does not depend on
+ * it; the processing order does, and with it the work.  No counterpart in
+ * the reference. */
+int fim_set_window(fim_ctx *c, int32_t inst, double window);
 /* converge mode: initial upper-bound field (global, nn values) with every
  * vertex active (warm start) */
 int fim_init_field(fim_ctx *c, int32_t inst, const float *phi0);
@@ -356,6 +363,13 @@ int fim_nn_predict(fim_nn *nn, const float *x, int64_t n, float *t);
 /* prediction for every vertex of the creation mesh for one solve (us) */
 int fim_nn_predict_map(fim_nn *nn, const int64_t *sites, int32_t n_sites, const float *scales,
                        float *t);
+/* the latest predicted activation time (us) of each of `count` solves over
+ * a vertex sample: out[j] = max over sample[0..m) of the prediction for
+ * solve j (sites: count x 3 ids, -1 pads; scales: count x 4).  Solves are
+ * batched into forward passes of whole samples.  Used to give each instance
+ * of a batch its own time window (fim_set_window). */
+int fim_nn_predict_max(fim_nn *nn, int32_t count, const int64_t *sites, const float *scales,
+                       const int64_t *sample, int32_t m, float *out);
 /* warm start of instance inst of c: field = prediction + delta (us),
  * sites = 0; all_active marks every vertex active (else only the times are
  * set, and the caller seeds the sites) */

@@ -32,6 +32,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -593,13 +594,29 @@ static void compute_step(sub_t *s, int variant, double fw, double cur_time,
 
 static int compat_solve(const orc_mesh *m, const orc_events *ev, double dt, int64_t n_steps,
                         double apd, double front_width, int variant, double *phi_out,
-                        int32_t *state_out, double *stats, int timed);
+                        int32_t *state_out, double *stats, int timed, int threads,
+                        double *seconds);
 
 int orc_compat_solve(const orc_mesh *m, const orc_events *ev, double dt,
                      int64_t n_steps, double apd, double front_width,
                      int variant, double *phi_out, int32_t *state_out,
                      double *stats) {
-  return compat_solve(m, ev, dt, n_steps, apd, front_width, variant, phi_out, state_out, stats, 0);
+  return compat_solve(m, ev, dt, n_steps, apd, front_width, variant, phi_out, state_out, stats, 0,
+                      1, NULL);
+}
+
+/* The same solve with the step loop on `threads` host threads (OpenMP):
+ * the interface merge over global vertices and compute_step over submeshes,
+ * one submesh per thread at a time, as the reference runs one block per
+ * submesh.  Submeshes share nothing within a compute phase, so the result
+ * is bit-identical to orc_compat_solve.  seconds[0..2] (may be NULL)
+ * receive the setup, step-loop and merge times; the step loop alone is what
+ * the reference times (cuda_fim.cu:2424-2447). */
+int orc_compat_solve_par(const orc_mesh *m, const orc_events *ev, double dt, int64_t n_steps,
+                         double apd, double front_width, int variant, int threads,
+                         double *phi_out, int32_t *state_out, double *seconds) {
+  return compat_solve(m, ev, dt, n_steps, apd, front_width, variant, phi_out, state_out, NULL, 0,
+                      threads > 0 ? threads : 1, seconds);
 }
 
 /* Timed events: the same windows, but event i fires in the window
@@ -613,12 +630,15 @@ int orc_compat_solve(const orc_mesh *m, const orc_events *ev, double dt,
 int orc_compat_solve_timed(const orc_mesh *m, const orc_events *ev, double dt, int64_t n_steps,
                            double apd, double front_width, int variant, double *phi_out,
                            int32_t *state_out, double *stats) {
-  return compat_solve(m, ev, dt, n_steps, apd, front_width, variant, phi_out, state_out, stats, 1);
+  return compat_solve(m, ev, dt, n_steps, apd, front_width, variant, phi_out, state_out, stats, 1,
+                      1, NULL);
 }
 
 static int compat_solve(const orc_mesh *m, const orc_events *ev, double dt, int64_t n_steps,
                         double apd, double front_width, int variant, double *phi_out,
-                        int32_t *state_out, double *stats, int timed) {
+                        int32_t *state_out, double *stats, int timed, int threads,
+                        double *seconds) {
+  const double t_setup0 = omp_get_wtime();
   const int64_t ne = m->ne, nn = m->nn;
   if (ne <= 0 || nn <= 0 || !m->atags) return -1;
   int64_t *e2n = (int64_t *)malloc(sizeof(int64_t) * 4 * ne);
@@ -638,21 +658,30 @@ static int compat_solve(const orc_mesh *m, const orc_events *ev, double dt, int6
   sub_t *subs = (sub_t *)calloc(nagg, sizeof(sub_t));
   int64_t *g2l = (int64_t *)malloc(sizeof(int64_t) * nn);
   for (int64_t v = 0; v < nn; ++v) g2l[v] = -1;
+  /* elements of each aggregate in ascending order (a counting sort by e2a) */
+  int64_t *a_dsp = (int64_t *)calloc(nagg + 1, sizeof(int64_t));
+  int64_t *a_el = (int64_t *)malloc(sizeof(int64_t) * ne);
+  for (int64_t e = 0; e < ne; ++e) a_dsp[e2a[e] + 1]++;
+  for (int64_t a = 0; a < nagg; ++a) a_dsp[a + 1] += a_dsp[a];
+  {
+    int64_t *fill = (int64_t *)calloc(nagg, sizeof(int64_t));
+    for (int64_t e = 0; e < ne; ++e) a_el[a_dsp[e2a[e]] + fill[e2a[e]]++] = e;
+    free(fill);
+  }
   for (int64_t a = 0; a < nagg; ++a) {
     sub_t *s = subs + a;
     s->nn = l2g_dsp[a + 1] - l2g_dsp[a];
     s->l2g = (int64_t *)malloc(sizeof(int64_t) * s->nn);
     memcpy(s->l2g, l2g + l2g_dsp[a], sizeof(int64_t) * s->nn);
     for (int64_t l = 0; l < s->nn; ++l) g2l[s->l2g[l]] = l;
-    int64_t cnt = 0;
-    for (int64_t e = 0; e < ne; ++e) cnt += (e2a[e] == a);
+    const int64_t cnt = a_dsp[a + 1] - a_dsp[a];
     s->ne = cnt;
     s->nenc = 4 * cnt;
     s->e2n = (int64_t *)malloc(sizeof(int64_t) * 4 * cnt);
     s->geo = (double *)malloc(sizeof(double) * 6 * cnt);
     int64_t le = 0;
-    for (int64_t e = 0; e < ne; ++e) {
-      if (e2a[e] != a) continue;
+    for (int64_t q = a_dsp[a]; q < a_dsp[a + 1]; ++q) {
+      const int64_t e = a_el[q];
       for (int k = 0; k < 4; ++k) s->e2n[4 * le + k] = g2l[e2n[4 * e + k]];
       orc_sort_nodes(s->e2n + 4 * le, 4); /* submesh compute_base_connectivity */
       int64_t gcon[4];
@@ -685,6 +714,8 @@ static int compat_solve(const orc_mesh *m, const orc_events *ev, double dt, int6
     s->snap = (int32_t *)malloc(sizeof(int32_t) * s->nn);
     for (int64_t l = 0; l < s->nn; ++l) g2l[s->l2g[l]] = -1;
   }
+  free(a_dsp);
+  free(a_el);
 
   /* n2i: copies of each global vertex, ascending aggregate (types.cpp:538-545) */
   int64_t *ci_cnt = (int64_t *)calloc(nn + 1, sizeof(int64_t));
@@ -722,6 +753,7 @@ static int compat_solve(const orc_mesh *m, const orc_events *ev, double dt, int6
       }
     }
   if (stats) stats[0] = stats[1] = 0.0;
+  const double t_loop0 = omp_get_wtime();
   double cur = 0.0;
   for (int64_t step = 0; step < n_steps; ++step) {
     const double phi_start = cur;
@@ -738,7 +770,9 @@ static int compat_solve(const orc_mesh *m, const orc_events *ev, double dt, int6
           }
         }
       }
-    /* sync_step2 (cuda_fim.cu:1752-1787), per global vertex */
+    /* sync_step2 (cuda_fim.cu:1752-1787), per global vertex; a vertex's
+     * copies are its own, so vertices run in parallel */
+#pragma omp parallel for num_threads(threads) schedule(static, 4096) if (threads > 1)
     for (int64_t v = 0; v < nn; ++v) {
       double mn = DMAX;
       for (int64_t p = ci_cnt[v]; p < ci_cnt[v + 1]; ++p) {
@@ -753,9 +787,11 @@ static int compat_solve(const orc_mesh *m, const orc_events *ev, double dt, int6
         }
       }
     }
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) if (threads > 1)
     for (int64_t a = 0; a < nagg; ++a)
       compute_step(subs + a, variant, fw, cur, phi_start, dt, apd, stats);
   }
+  const double t_loop1 = omp_get_wtime();
   /* host merge (cuda_fim.cu:2464-2485) */
   for (int64_t v = 0; v < nn; ++v) {
     phi_out[v] = DMAX;
@@ -768,6 +804,11 @@ static int compat_solve(const orc_mesh *m, const orc_events *ev, double dt, int6
       if (subs[a].state[l] > state_out[g]) state_out[g] = subs[a].state[l];
     }
 
+  if (seconds) {
+    seconds[0] = t_loop0 - t_setup0;
+    seconds[1] = t_loop1 - t_loop0;
+    seconds[2] = omp_get_wtime() - t_loop1;
+  }
   for (int64_t a = 0; a < nagg; ++a) sub_free(subs + a);
   free(subs); free(g2l); free(ci_cnt); free(ci_a); free(ci_l); free(ci_fill);
   free(l2g); free(l2g_dsp); free(e2a); free(e2n);

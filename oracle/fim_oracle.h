@@ -145,6 +145,12 @@ int orc_compat_solve(const orc_mesh *m, const orc_events *ev, double dt,
 
 /* Same windows with timed events: event i fires in the window that holds
  * its tstart (phi = tstart), every active event fires. */
+/* orc_compat_solve with the step loop on `threads` host threads (OpenMP),
+ * bit-identical to it; seconds[0..2] = setup, step loop, merge (may be
+ * NULL).  Events fire at t = 0 as in orc_compat_solve. */
+int orc_compat_solve_par(const orc_mesh *m, const orc_events *ev, double dt, int64_t n_steps,
+                         double apd, double front_width, int variant, int threads,
+                         double *phi_out, int32_t *state_out, double *seconds);
 int orc_compat_solve_timed(const orc_mesh *m, const orc_events *ev, double dt,
                            int64_t n_steps, double apd, double front_width,
                            int variant, double *phi_out, int32_t *state_out,

@@ -72,6 +72,10 @@ def lib():
         L.orc_compat_solve.argtypes = [C.POINTER(_Mesh), C.POINTER(_Events), C.c_double,
                                        C.c_int64, C.c_double, C.c_double, C.c_int,
                                        _pd, _pi32, _pd]
+        L.orc_compat_solve_par.restype = C.c_int
+        L.orc_compat_solve_par.argtypes = [C.POINTER(_Mesh), C.POINTER(_Events), C.c_double,
+                                           C.c_int64, C.c_double, C.c_double, C.c_int, C.c_int,
+                                           _pd, _pi32, _pd]
         L.orc_compat_solve_timed.restype = C.c_int
         L.orc_compat_solve_timed.argtypes = [C.POINTER(_Mesh), C.POINTER(_Events), C.c_double,
                                        C.c_int64, C.c_double, C.c_double, C.c_int,
@@ -217,6 +221,23 @@ def compat_solve(mesh: Mesh, stims, dt, n_steps, apd=3e5, front_width=-1.0,
         raise RuntimeError(f"orc_compat_solve failed: {rc}")
     del keep
     return phi, st, stats
+
+
+def compat_solve_par(mesh: Mesh, stims, dt, n_steps, threads, apd=3e5, front_width=-1.0,
+                     variant=LOCAL_REF):
+    """compat_solve with the step loop on `threads` host threads; returns
+    (phi, states, seconds) with seconds = (setup, step loop, merge)."""
+    ev, keep = _events(stims)
+    nn = mesh.xyz.shape[0]
+    phi = np.zeros(nn)
+    st = np.zeros(nn, dtype=np.int32)
+    sec = np.zeros(3)
+    rc = lib().orc_compat_solve_par(C.byref(mesh.c), C.byref(ev), dt, n_steps, apd, front_width,
+                                    variant, threads, _p(phi, _pd), _p(st, _pi32), _p(sec, _pd))
+    if rc != 0:
+        raise RuntimeError(f"orc_compat_solve_par failed: {rc}")
+    del keep
+    return phi, st, sec
 
 
 def fim_solve(mesh: Mesh, src, t0=None, variant=LOCAL_EXACT):

@@ -101,28 +101,30 @@ def test_cli_unit_test_configuration(tmp_path):
     assert out.read_bytes() == ref.read_bytes()
 
 
-def test_first_inactive_event_stops_the_seeding(tmp_path):
-    """update_events_gpu breaks at the first event that is inactive
-    (cuda_fim.cu:636-640): with an inactive stimulus ordered before S0 (the
-    plan's stimuli are read in key order) nothing is seeded and every
-    vertex stays far / unreached"""
-    def edit(plan):
+def test_event_order_and_the_first_miss(tmp_path):
+    """The plan's stimuli are read in key order and inactive ones are dropped
+    while parsing (io_json.cpp:154-160), so an inactive stimulus ordered
+    before S0 changes nothing.  update_events_gpu then seeds the events in
+    order at cur_time 0 and stops at the first one that does not start at or
+    after it (cuda_fim.cu:636-640): a stimulus with a negative start ordered
+    before S0 leaves every vertex far and unreached."""
+    def inactive_first(plan):
         st = plan["simulation"]["stimuli"]
         st["A_off"] = dict(st["S0"], active=0)
 
     out = tmp_path / "off.vtk"
-    F.run_reference(MESH, TAGS, _plan(tmp_path, edit=edit), str(out), reps=1, cfg=CFG)
-    pd = vtkio.read_vtk(str(out))["point_data"]
-    assert np.all(pd["states"] == F.FAR) and np.all(pd["phi"] == -1.0)
+    F.run_reference(MESH, TAGS, _plan(tmp_path, edit=inactive_first), str(out), reps=1, cfg=CFG)
+    _check_against_golden(str(out))
 
-    # an active event ordered first seeds as well: the same map as S0 alone
-    def edit2(plan):
+    def negative_first(plan):
         st = plan["simulation"]["stimuli"]
-        st["A_on"] = dict(st["S0"], active=1)
+        st["A_neg"] = json.loads(json.dumps(st["S0"]))
+        st["A_neg"]["protocol"]["start"] = -1.0
 
-    out2 = tmp_path / "on.vtk"
-    F.run_reference(MESH, TAGS, _plan(tmp_path, edit=edit2), str(out2), reps=1, cfg=CFG)
-    _check_against_golden(str(out2))
+    out2 = tmp_path / "neg.vtk"
+    F.run_reference(MESH, TAGS, _plan(tmp_path, edit=negative_first), str(out2), reps=1, cfg=CFG)
+    pd = vtkio.read_vtk(str(out2))["point_data"]
+    assert np.all(pd["states"] == F.FAR) and np.all(pd["phi"] == -1.0)
 
 
 ADAPTER = os.path.join(ROOT, "oracle", "_ref", "adapter_run")

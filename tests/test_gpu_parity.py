@@ -301,6 +301,31 @@ def test_converge_time_window_same_fixed_point(window, staged):
     assert res < 1e-5
 
 
+def test_per_instance_windows_same_fixed_point():
+    """fim_set_window: four instances of one batch with windows 0 (plain FIM
+    order), 0.5, 1 and 4 ms reach the oracle's fixed point, and each equals
+    the same solve run alone with that window as the context's, bit for bit"""
+    cfg = cases.scaled(cases.SLAB, 4)
+    m, om = cases.box(cfg, seed=2)
+    p = F.Pack(m, F.PART_RCB, max_nn=512, seed=2)
+    src = cases.pacing_sites(cfg, 5, seed=2)
+    wins = (0.0, 500.0, 1000.0, 4000.0)
+    s = F.Solver(p, n_instances=len(wins), window=1000.0)
+    for i, w in enumerate(wins):
+        s.set_window(w, inst=i)
+        s.seed(src, 0.0, inst=i)
+    s.run()
+    maps = [s.download(inst=i)[0] for i in range(len(wins))]
+    with pytest.raises(F.FimError):
+        s.set_window(-1.0, inst=0)
+    s.close()
+    ref, _ = O.fim_solve(om, src, variant=O.LOCAL_EXACT)
+    for i, w in enumerate(wins):
+        _check_fp32(maps[i], ref)
+        one, *_ = _converge(m, p, src, window=w)
+        assert np.array_equal(one, maps[i]), w
+
+
 def test_warm_start_from_upper_bound_reaches_same_fixed_point():
     """init_field with every vertex active: an upper bound (oracle + 5 %)
     converges to the cold-start fixed point"""

@@ -92,6 +92,7 @@ def test_adam_steps_match_oracle(small_slab):
     rng = np.random.default_rng(4)
     b1, b2, eps = float(np.float32(0.9)), float(np.float32(0.999)), float(np.float32(1e-8))
     p, mo, vo = p0.astype(np.float64), np.zeros(M.NPARAMS), np.zeros(M.NPARAMS)
+    gsum, g2sum = np.zeros(M.NPARAMS), np.zeros(M.NPARAMS)  # scales of the moments' FP32 rounding
     for t in (1, 2, 3):
         g = (rng.normal(0, 1e-2, M.NPARAMS) * rng.choice([1e-3, 1.0, 1e2], M.NPARAMS)).astype(np.float32)
         g[::97] = 0.0
@@ -99,13 +100,18 @@ def test_adam_steps_match_oracle(small_slab):
             g[:5000] = -g[:5000]
         nn.adam_step(g)
         p, mo, vo = M.adam_step(p, mo, vo, g.astype(np.float64), t, lr, b1, b2, eps)
+        gsum += np.abs(g)
+        g2sum += g.astype(np.float64) ** 2
     got = nn.params().astype(np.float64)
     gm, gv, steps = nn.moments()
     assert steps == 3
     tol = 4 * np.spacing(np.abs(p).astype(np.float32)).astype(np.float64) + 1e-5 * lr
     assert np.all(np.abs(got - p) <= tol), float(np.max(np.abs(got - p) - tol))
-    assert np.allclose(gm, mo, rtol=1e-5, atol=1e-12)
-    assert np.allclose(gv, vo, rtol=1e-5, atol=1e-18)
+    # moments to FP32 accumulation (a few roundings of the terms summed;
+    # m can cancel to near zero after a sign change, so the bound is on the
+    # magnitudes that were summed, not on m itself)
+    assert np.all(np.abs(gm - mo) <= 1e-6 * gsum + 1e-30), float(np.max(np.abs(gm - mo) - 1e-6 * gsum))
+    assert np.all(np.abs(gv - vo) <= 1e-6 * g2sum + 1e-30), float(np.max(np.abs(gv - vo) - 1e-6 * g2sum))
     assert np.max(np.abs(got - p0)) > 0.1 * lr  # the steps moved the parameters
     x = rng.uniform(-1, 1, (256, M.NIN)).astype(np.float32)
     y = nn.predict(x)

@@ -95,6 +95,30 @@ def test_golden_and_the_reference_tet_formula():
     assert replays(2) and replays(3)  # not distinguished by the golden
 
 
+def test_compat_on_host_threads_is_bit_identical():
+    """orc_compat_solve_par (the step loop on OpenMP threads, one submesh per
+    thread) equals the serial restatement bit for bit: on the golden case
+    and on a generated mesh with a partition of the build's own"""
+    xyz, cells, tags = _ut()
+    m = O.Mesh(xyz, cells, atags=tags)
+    a = O.compat_solve(m, cases.ut_plan_stimuli(), 150e3, 3)
+    b = O.compat_solve_par(m, cases.ut_plan_stimuli(), 150e3, 3, 4)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert b[2][1] >= 0.0
+    import fimpy as F
+    cfg = cases.scaled(cases.SLAB, 8)
+    fm, om = cases.box(cfg, seed=6)
+    e2s, _ = F.Pack(fm, F.PART_RCB, max_nn=128, seed=6).partition()
+    om = O.Mesh(om.xyz, om.e2n, fib=om.fib, vel=om.vel, atags=e2s.astype(np.int32))
+    stims = [(1, 0.0, [int(v) for v in cases.pacing_sites(cfg, 3, seed=6)])]
+    fw = fm.front_width()
+    a = O.compat_solve(om, stims, 2000.0, 100, front_width=fw)
+    for th in (2, 5):
+        b = O.compat_solve_par(om, stims, 2000.0, 100, th, front_width=fw)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.all(a[0] < 1e50)  # every vertex reached in 100 windows (before any recovers)
+
+
 # ------------------------------------------------- reference host code ----
 @need_ref
 def test_connectivity_matches_reference_host_code():
