@@ -296,6 +296,7 @@ class Solver:
         self.pack = pack
         self.nn = pack.info["nn_global"]
         self.n_instances = n_instances
+        self.window = window  # every instance's window until set_window changes it
         o = Opts(mode, local, n_instances, device, block, local_iters, graph_iters, grid, eps_rel, apd,
                  int(profile), int(staged), window, unit, waves)
         self.h = _vp()

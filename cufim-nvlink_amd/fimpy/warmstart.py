@@ -1,6 +1,16 @@
 """Surrogate warm start: training maps from the batched solver, training of
-the coordinate MLP, calibration of the upward shift delta, and warm-started
-batched solves judged by the support check with a cold re-solve as fallback.
+the coordinate MLP, and two ways to use it on a batch of new solves.
+
+  scheduled (the default, `scheduled_batch`): the surrogate predicts each
+    solve's latest activation time, and every instance of the batch gets its
+    own time window (fim_set_window) so that all of them finish in about the
+    same number of global iterations.  Windows change only the processing
+    order, never the fixed point, so no check of the result is needed.
+  upper bound (`warm_solve_all`): prediction + delta as the initial field of
+    every vertex, the support check as judge and a cold re-solve as
+    fallback.  Kept for comparison: under the window order the front lowers
+    every vertex below any certified upper bound before it acts, so this
+    prunes nothing.
 
 A solve is (sites, scales): 1-3 pacing vertices (time 0) and four regional
 speed factors.  All work runs through libcufim.so (fim_nn_*, fim_*).
@@ -33,11 +43,60 @@ def _chunks(seq, n):
         yield i, seq[i:i + n]
 
 
-def cold_batch(solver, solves):
-    """Cold batched solve of len(solves) instances; returns the solver stats."""
+def window_sample(mesh, k=2040, seed=0):
+    """Vertices at which the latest activation of a solve is estimated: the
+    vertex nearest each corner of the mesh's bounding box and k seeded
+    random others (the latest activation lies on the boundary, mostly at a
+    corner)."""
+    xyz, _, _ = mesh.arrays()
+    lo, hi = xyz.min(axis=0), xyz.max(axis=0)
+    span = np.where(hi > lo, hi - lo, 1.0)
+    corners = [int(np.argmin(np.sum(((xyz - np.array([a, b, c])) / span) ** 2, axis=1)))
+               for a in (lo[0], hi[0]) for b in (lo[1], hi[1]) for c in (lo[2], hi[2])]
+    rng = np.random.default_rng(seed)
+    rand = rng.choice(xyz.shape[0], min(k, xyz.shape[0]), replace=False)
+    return np.unique(np.concatenate([corners, rand])).astype(np.int64)
+
+
+def scheduled_windows(tmax, window, quantile=0.75, floor=250.0):
+    """Per-solve time windows (us per global iteration) under which the
+    solves of one batch need about the same number of iterations: the batch
+    takes K = quantile(tmax) / window iterations and solve j gets
+    tmax_j / K, at least `floor`.  A solve's window sets how much of its
+    front one iteration may process; the fixed point does not depend on it."""
+    tmax = np.asarray(tmax, dtype=np.float64)
+    k = max(1.0, float(np.quantile(tmax, quantile)) / window)
+    return np.maximum(floor, tmax / k)
+
+
+def scheduled_batch(nn: Surrogate, solver: Solver, solves, sample, window=2000.0, quantile=0.75):
+    """Surrogate-scheduled batched solve of len(solves) instances: predicted
+    latest activation per solve (one device call over the vertex sample),
+    per-instance windows, regional factors, seeds, then FIM to the fixed
+    point.  Returns (stats, seconds of prediction, seconds of the rest)."""
+    t0 = time.perf_counter()
+    tmax = nn.predict_max(solves, sample)
+    t1 = time.perf_counter()
+    wins = scheduled_windows(tmax, window, quantile)
     solver.reset()
     for i, (sites, sc) in enumerate(solves):
         solver.set_scaling(sc, inst=i)
+        solver.set_window(float(wins[i]), inst=i)
+    solver.seed_batch([s for s, _ in solves], 0.0)
+    solver.run(max_iters=1_000_000)
+    t2 = time.perf_counter()
+    return solver.stats(), t1 - t0, t2 - t1
+
+
+def cold_batch(solver, solves, window=None):
+    """Cold batched solve of len(solves) instances, every instance at one
+    window (the context's, or `window` when given); returns the solver
+    stats."""
+    w = solver.window if window is None else window
+    solver.reset()
+    for i, (sites, sc) in enumerate(solves):
+        solver.set_scaling(sc, inst=i)
+        solver.set_window(w, inst=i)
     solver.seed_batch([s for s, _ in solves], 0.0)
     solver.run(max_iters=1_000_000)
     return solver.stats()

@@ -7,7 +7,11 @@
     1-3 pacing sites and four quadrant speed factors U[0.5, 1.5] each: one
     instance per mesh against the oracle on that mesh with every element's
     speeds multiplied by its region's factor, and every instance confined to
-    its own component.
+    its own component;
+  the surrogate on the slab: trained on 4,096 batched-FIM maps, then the 512
+    held-out solves as one surrogate-scheduled batch -- every map within
+    tolerance of the cold batch's, two against the FP64 oracle, and fewer
+    global iterations than the cold batch (what the network gains).
 
 Tolerance: max |dT| <= 1e-4 * max(T) (FP32 against FP64), which on these
 meshes is well inside the 0.1 ms the throughput figure is quoted at; the
@@ -96,3 +100,48 @@ def test_four_mesh_union_256_solves_vs_fp64_oracle():
     for i in picks:
         k = solves[i]["part"]
         _check(got[i][bounds[k]:bounds[k + 1]], refs[i])
+
+
+def test_surrogate_scheduled_full_slab_512_held_out():
+    from fimpy import warmstart as WS
+    cfg = W.SLAB
+    m = W.box(cfg, 2)
+    reg = W.quadrant_regions(m)
+    m.set_regions(reg)
+    p = F.Pack(m, F.PART_RCB, max_nn=2816, seed=2)
+    train = WS.draw_solves(cfg, 4096, 1003)
+    test = WS.draw_solves(cfg, 512, 3005)
+    nn = F.Surrogate(m, seed=2, max_maps=4096, max_batch=65536)
+    WS.build_dataset(nn, p, train, batch=512, window=2000.0, local_iters=4, block=512)
+    assert nn.dataset_size() == 4096
+    l0 = nn.train(1, batch=65536, seed=1)
+    l1 = nn.train(4000, batch=65536, seed=2)
+    assert l1 < 0.05 * l0, (l0, l1)
+    s = F.Solver(p, n_instances=512, window=2000.0, unit=2, local_iters=4, block=512)
+    cold = WS.cold_batch(s, test)
+    cold_maps = [s.download(inst=i)[0] for i in range(512)]
+    sample = WS.window_sample(m)
+    st, _, _ = WS.scheduled_batch(nn, s, test, sample, window=2000.0, quantile=0.75)
+    # the network's gain: the batch needs fewer global iterations
+    assert st["global_iters"] < cold["global_iters"], (st["global_iters"], cold["global_iters"])
+    for i in range(512):  # windows change the order only: the cold maps within tolerance
+        _check(s.download(inst=i)[0], cold_maps[i])
+    got = {i: s.download(inst=i)[0] for i in (0, 511)}
+    # the surrogate alone is a usable but coarse map
+    pred = nn.predict_map(*test[0])
+    assert np.mean(np.abs(pred - cold_maps[0])) < 0.05 * cold_maps[0].max()
+    s.close()
+    nn.close()
+    xyz, e2n, _ = m.arrays()
+    fib, vel = m.material()
+
+    def oracle(i):
+        sites, sc = test[i]
+        om = O.Mesh(xyz, e2n, fib=fib, vel=vel * sc.astype(np.float64)[reg][:, None])
+        return O.fim_solve(om, sites, variant=O.LOCAL_EXACT)[0]
+
+    with ThreadPoolExecutor(2) as ex:
+        refs = dict(zip((0, 511), ex.map(oracle, (0, 511))))
+    for i in (0, 511):
+        _check(got[i], refs[i], abs_tol=100.0)
+

@@ -196,6 +196,25 @@ def test_training_and_warm_start_reach_the_cold_fixed_point(small_slab):
     nn.close()
 
 
+def test_predicted_maximum_equals_the_map_maximum(small_slab):
+    """fim_nn_predict_max over a vertex sample equals the largest value of
+    fim_nn_predict_map at those vertices (the same rows through the same
+    kernels, so bit for bit), for several solves in one call"""
+    from fimpy import warmstart as WS
+    cfg, m = small_slab
+    nn = F.Surrogate(m, seed=3)
+    nn.set_params(M.init_params(11))
+    solves = _draw(cfg, 37, 5)
+    sample = WS.window_sample(m, k=500, seed=1)
+    got = nn.predict_max(solves, sample)
+    for j in (0, 17, 36):
+        t = nn.predict_map(*solves[j])
+        assert got[j] == t[sample].max(), j
+    with pytest.raises(F.FimError):
+        nn.predict_max(solves, np.array([m.nn], dtype=np.int64))
+    nn.close()
+
+
 def test_support_check_flags_a_field_below_the_fixed_point(small_slab):
     cfg, m = small_slab
     p = F.Pack(m, F.PART_RCB, max_nn=256, seed=3)
