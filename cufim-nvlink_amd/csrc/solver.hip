@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1598,6 +1599,14 @@ T *dupload(const std::vector<T> &v, std::vector<void *> &owned) {
 }  // namespace
 }  // namespace fim
 
+// Scratch of fim_support_noalloc: the best candidate time (float bits) of
+// every vertex, for meshes of up to kNoAllocMaxVertices vertices (4 MiB).
+// One copy per device, held until the process ends.  Host code never uses
+// this name as an address: fim_create looks the device address up once per
+// device (noalloc_scratch) and the kernels get that pointer.
+constexpr uint32_t kNoAllocMaxVertices = 1u << 20;
+static __device__ __align__(256) uint32_t scratch[kNoAllocMaxVertices];
+
 using namespace fim;
 
 struct fim_ctx {
@@ -1864,6 +1873,23 @@ void ctx_init_field(fim_ctx *c, int32_t inst, bool all_active) {
   HIPCHECK(hipGetLastError());
 }
 
+// device address of `scratch` on the current device `dev`, looked up once
+// per device (hipGetSymbolAddress) and kept
+uint32_t *noalloc_scratch(int dev) {
+  static std::mutex mu;
+  static std::vector<uint32_t *> ptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (dev < 0) throw Error(FIM_EINVAL, "bad device ordinal");
+  if ((size_t)dev >= ptr.size()) ptr.resize((size_t)dev + 1, nullptr);
+  if (!ptr[dev]) {
+    void *p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(FIM_EDEVICE, "no device address for the support scratch");
+    ptr[dev] = static_cast<uint32_t *>(p);
+  }
+  return ptr[dev];
+}
+
 // support check of one instance with `best` (nn words of device memory) as
 // the per-vertex candidate minimum
 double support_into(fim_ctx *c, int32_t inst, uint32_t *best) {
@@ -1897,6 +1923,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
       c->o = *o;
       c->dev = o->device;
       HIPCHECK(hipSetDevice(c->dev));
+      (void)noalloc_scratch(c->dev);  // once per device: fim_support_noalloc's scratch
       HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       HIPCHECK(hipEventCreate(&c->ev0));
       HIPCHECK(hipEventCreate(&c->ev1));
@@ -2526,6 +2553,20 @@ int fim_support(fim_ctx *c, int32_t inst, double *worst) {
       throw;
     }
     HIPCHECK(hipFree(best));
+  });
+}
+
+int fim_support_noalloc(fim_ctx *c, int32_t inst, double *worst) {
+  return guarded([&] {
+    if (!c || !worst) throw Error(FIM_EINVAL, "null argument");
+    if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "support check needs converge mode");
+    check_inst(c, inst);
+    if (c->nn > kNoAllocMaxVertices)
+      throw Error(FIM_ELIMIT, "fim_support_noalloc: " + std::to_string(c->nn) +
+                                  " vertices exceed its scratch of " + std::to_string(kNoAllocMaxVertices) +
+                                  " (use fim_support)");
+    HIPCHECK(hipSetDevice(c->dev));
+    *worst = support_into(c, inst, noalloc_scratch(c->dev));
   });
 }
 

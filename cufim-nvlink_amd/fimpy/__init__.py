@@ -122,6 +122,7 @@ SIGNATURES = {
     "fim_download": (C.c_int, [_vp, C.c_int32, _pd, _pi32]),
     "fim_residual": (C.c_int, [_vp, C.c_int32, _pd]),
     "fim_support": (C.c_int, [_vp, C.c_int32, _pd]),
+    "fim_support_noalloc": (C.c_int, [_vp, C.c_int32, _pd]),
     "fim_nn_create": (C.c_int, [_vp, C.POINTER(NNOpts), C.POINTER(_vp)]),
     "fim_nn_destroy": (None, [_vp]),
     "fim_nn_get_params": (C.c_int, [_vp, _pf]),
@@ -378,6 +379,12 @@ class Solver:
     def support(self, inst=0):
         r = C.c_double()
         _check(lib().fim_support(self.h, inst, C.byref(r)))
+        return r.value
+
+    def support_noalloc(self, inst=0):
+        """support() through the entry point that allocates nothing"""
+        r = C.c_double()
+        _check(lib().fim_support_noalloc(self.h, inst, C.byref(r)))
         return r.value
 
     def stats(self):

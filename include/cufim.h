@@ -294,6 +294,14 @@ int fim_residual(fim_ctx *c, int32_t inst, double *res);
  * larger value means some vertex started below its fixed-point time (a warm
  * start that was not an upper bound) and the map must be recomputed. */
 int fim_support(fim_ctx *c, int32_t inst, double *worst);
+/* fim_support without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops.  The per-vertex candidate minima live in a
+ * static device array of the library, 2^20 vertices (4 MiB) per device, so
+ * meshes with more vertices are rejected (FIM_ELIMIT; use fim_support).  All
+ * contexts on one device share that array: calls that overlap on one GPU
+ * (different host threads, or streams without ordering) race. */
+int fim_support_noalloc(fim_ctx *c, int32_t inst, double *worst);
 int fim_stats(const fim_ctx *c, fim_stats_t *s);
 void fim_destroy(fim_ctx *c);
 
