@@ -51,6 +51,10 @@ constexpr size_t NPAD = OFF_BO + 1;
 // BF16 weight copies
 constexpr size_t offb_w(int l) { return l == 0 ? 0 : W0_N + (size_t)(l - 1) * WL_N; }
 constexpr size_t NWB = W0_N + (size_t)(NL - 1) * WL_N;
+// BF16 copies of the hidden layers transposed (WT_l[k][n] = W_l[n][k], l >= 1):
+// the backward pass reads them contiguous in its K dimension
+constexpr size_t offbt(int l) { return (size_t)(l - 1) * WL_N; }
+constexpr size_t NWBT = (size_t)(NL - 1) * WL_N;
 static_assert(off_w(NL) == off_b(NL - 1) + HID, "layout");
 static_assert((size_t)HID * NIN + HID + (NL - 1) * (WL_N + HID) + HID + 1 == FIM_NN_PARAMS,
               "logical parameter count");
@@ -166,38 +170,45 @@ __global__ __launch_bounds__(256) void k_gemm128(const uint16_t *__restrict__ A,
       if (AK) {
         const uint32_t r = (tid >> 2) + 64u * j, kq = (tid & 3u) * 8u;
         ra[j] = *reinterpret_cast<const u16x8 *>(A + (r0 + r) * sar + k0 + kq);
-      } else {
-        const uint32_t k = (tid >> 4) + 16u * j, rq = (tid & 15u) * 8u;
+      } else {  // rows k0 + 2 (tid >> 4) + j: two neighbouring k per thread
+        const uint32_t k = 2u * (tid >> 4) + j, rq = (tid & 15u) * 8u;
         ra[j] = *reinterpret_cast<const u16x8 *>(A + (k0 + k) * sak + r0 + rq);
       }
       if (BK) {
         const uint32_t c = (tid >> 2) + 64u * j, kq = (tid & 3u) * 8u;
         rb[j] = *reinterpret_cast<const u16x8 *>(B + (c0 + c) * sbc + k0 + kq);
       } else {
-        const uint32_t k = (tid >> 4) + 16u * j, cq = (tid & 15u) * 8u;
+        const uint32_t k = 2u * (tid >> 4) + j, cq = (tid & 15u) * 8u;
         rb[j] = *reinterpret_cast<const u16x8 *>(B + (k0 + k) * sbk + c0 + cq);
       }
     }
   };
+  // transposed staging writes the two neighbouring k of a row as one
+  // 32-bit word (k is even, rows are 80 B apart, so the word is aligned)
   auto lstore = [&](int buf) {
+    if (AK) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      if (AK) {
+      for (int j = 0; j < 2; ++j) {
         const uint32_t r = (tid >> 2) + 64u * j, kq = (tid & 3u) * 8u;
         *reinterpret_cast<u16x8 *>(&As[buf][r][kq]) = ra[j];
-      } else {
-        const uint32_t k = (tid >> 4) + 16u * j, rq = (tid & 15u) * 8u;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) As[buf][rq + i][k] = ra[j][i];
       }
-      if (BK) {
+    } else {
+      const uint32_t k = 2u * (tid >> 4), rq = (tid & 15u) * 8u;
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        *reinterpret_cast<uint32_t *>(&As[buf][rq + i][k]) = (uint32_t)ra[0][i] | ((uint32_t)ra[1][i] << 16);
+    }
+    if (BK) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
         const uint32_t c = (tid >> 2) + 64u * j, kq = (tid & 3u) * 8u;
         *reinterpret_cast<u16x8 *>(&Bs[buf][c][kq]) = rb[j];
-      } else {
-        const uint32_t k = (tid >> 4) + 16u * j, cq = (tid & 15u) * 8u;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) Bs[buf][cq + i][k] = rb[j][i];
       }
+    } else {
+      const uint32_t k = 2u * (tid >> 4), cq = (tid & 15u) * 8u;
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        *reinterpret_cast<uint32_t *>(&Bs[buf][cq + i][k]) = (uint32_t)rb[0][i] | ((uint32_t)rb[1][i] << 16);
     }
   };
   f32x16 acc[2][2];
@@ -403,18 +414,41 @@ __global__ __launch_bounds__(256) void k_head_infer(const uint16_t *H5, const fl
 }
 
 // column sums of dZ (B x 256) into g_b (bias gradient)
+// column sums of dZ (B x 256) into g_b: a thread sums 8 columns of every
+// eighth row of the block's rows (16-byte loads), the block reduces in LDS
 __global__ __launch_bounds__(256) void k_colsum(const uint16_t *dZ, uint32_t B, float *g_b) {
-  float s = 0.0f;
-  for (uint32_t row = blockIdx.x; row < B; row += gridDim.x) s += bf2f(dZ[(size_t)row * HID + threadIdx.x]);
-  atomicAdd(&g_b[threadIdx.x], s);
+  __shared__ float part[8][HID];
+  const uint32_t tid = threadIdx.x, cg = tid & 31u, rl = tid >> 5;  // column group, row lane
+  float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t row = blockIdx.x * 8u + rl; row < B; row += gridDim.x * 8u) {
+    const u16x8 v = *reinterpret_cast<const u16x8 *>(dZ + (size_t)row * HID + cg * 8u);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s[i] += bf2f(v[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) part[rl][cg * 8u + i] = s[i];
+  __syncthreads();
+  float t = 0.0f;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) t += part[r][tid];
+  atomicAdd(&g_b[tid], t);
 }
 
 // sum of the batch-split partials
+// (n a multiple of 4: 16-byte loads and stores)
 __global__ void k_reduce_parts(const float *P, uint32_t S, size_t n, float *g) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    float s = 0.0f;
-    for (uint32_t k = 0; k < S; ++k) s += P[(size_t)k * n + i];
-    g[i] = s;
+  const size_t n4 = n / 4;
+  const float4 *P4 = reinterpret_cast<const float4 *>(P);
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (uint32_t k = 0; k < S; ++k) {
+      const float4 v = P4[(size_t)k * n4 + i];
+      s.x += v.x;
+      s.y += v.y;
+      s.z += v.z;
+      s.w += v.w;
+    }
+    reinterpret_cast<float4 *>(g)[i] = s;
   }
 }
 
@@ -422,7 +456,7 @@ __global__ void k_reduce_parts(const float *P, uint32_t S, size_t n, float *g) {
 struct AdamArgs {
   float *w, *m, *v;
   const float *g;
-  uint16_t *wb;
+  uint16_t *wb, *wbt;
   float lr, b1, b2, eps, c1, c2;  // c1 = 1 - b1^t, c2 = 1 - b2^t
 };
 
@@ -437,14 +471,22 @@ __global__ void k_adam(AdamArgs a) {
     a.w[i] = w;
     // weights of layer l live at off_w(l) .. off_b(l) in the masters
     for (int l = 0; l < NL; ++l)
-      if (i >= off_w(l) && i < off_b(l)) a.wb[offb_w(l) + (i - off_w(l))] = f2bf(w);
+      if (i >= off_w(l) && i < off_b(l)) {
+        const size_t q = i - off_w(l);
+        a.wb[offb_w(l) + q] = f2bf(w);
+        if (l > 0) a.wbt[offbt(l) + (q % HID) * HID + q / HID] = f2bf(w);
+      }
   }
 }
 
-__global__ void k_to_bf16(const float *w, uint16_t *wb) {
+__global__ void k_to_bf16(const float *w, uint16_t *wb, uint16_t *wbt) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < NPAD; i += (size_t)gridDim.x * blockDim.x)
     for (int l = 0; l < NL; ++l)
-      if (i >= off_w(l) && i < off_b(l)) wb[offb_w(l) + (i - off_w(l))] = f2bf(w[i]);
+      if (i >= off_w(l) && i < off_b(l)) {
+        const size_t q = i - off_w(l);
+        wb[offb_w(l) + q] = f2bf(w[i]);
+        if (l > 0) wbt[offbt(l) + (q % HID) * HID + q / HID] = f2bf(w[i]);
+      }
 }
 
 __global__ void k_zero_sites(const uint32_t *v, uint32_t n, float *f) {
@@ -476,7 +518,7 @@ struct fim_nn {
   double ctr[3] = {0, 0, 0}, half = 1.0;
   // parameters
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
-  uint16_t *wb = nullptr;
+  uint16_t *wb = nullptr, *wbt = nullptr;
   int64_t t = 0;  // Adam steps taken
   // activations (rows = cap)
   uint32_t cap = 0;
@@ -567,17 +609,19 @@ void backward(fim_nn *n, uint32_t rows) {
     else
       k_gemm<false, false, EpiPart><<<dim3(K / 64, HID / 64, S), 256, 0, n->stream>>>(dz, 1, HID, hin, K, 1, span,
                                                                                        ep);
-    k_reduce_parts<<<grid_for((size_t)HID * K), 256, 0, n->stream>>>(n->parts, S, (size_t)HID * K, n->g + off_w(l));
-    k_colsum<<<std::min<uint32_t>(rows, 1024), 256, 0, n->stream>>>(dz, rows, n->g + off_b(l));
+    k_reduce_parts<<<grid_for((size_t)HID * K / 4), 256, 0, n->stream>>>(n->parts, S, (size_t)HID * K,
+                                                                         n->g + off_w(l));
+    k_colsum<<<std::min<uint32_t>(rows / 8, 512), 256, 0, n->stream>>>(dz, rows, n->g + off_b(l));
     if (l > 0) {
       // dZ_{l-1}[m][k] = sum_n dZ[m][n] W_l[n][k] * SiLU'(Z_{l-1})
       EpiBwd eb{n->Z[l - 1], dzp};
+      // B(k = n, c = k') = W_l[n][k'] = WT_l[k'][n]: contiguous in k
       if (rows % 128 == 0)
-        k_gemm128<true, false, EpiBwd><<<dim3(HID / 128, rows / 128, 1), 256, 0, n->stream>>>(
-            dz, HID, 1, n->wb + offb_w(l), HID, 1, HID, eb);
+        k_gemm128<true, true, EpiBwd><<<dim3(HID / 128, rows / 128, 1), 256, 0, n->stream>>>(
+            dz, HID, 1, n->wbt + offbt(l), 1, HID, HID, eb);
       else
-        k_gemm<true, false, EpiBwd><<<dim3(HID / 64, rows / 64, 1), 256, 0, n->stream>>>(
-            dz, HID, 1, n->wb + offb_w(l), HID, 1, HID, eb);
+        k_gemm<true, true, EpiBwd><<<dim3(HID / 64, rows / 64, 1), 256, 0, n->stream>>>(
+            dz, HID, 1, n->wbt + offbt(l), 1, HID, HID, eb);
       std::swap(dz, dzp);
     }
   }
@@ -592,6 +636,7 @@ void adam(fim_nn *n) {
   a.v = n->v;
   a.g = n->g;
   a.wb = n->wb;
+  a.wbt = n->wbt;
   a.lr = (float)(n->o.lr > 0 ? n->o.lr : 1e-3);
   a.b1 = (float)(n->o.beta1 > 0 ? n->o.beta1 : 0.9);
   a.b2 = (float)(n->o.beta2 > 0 ? n->o.beta2 : 0.999);
@@ -726,11 +771,12 @@ int fim_nn_create(const fim_mesh *mesh, const fim_nn_opts *o, fim_nn **out) {
       n->m = nn_alloc<float>(n, NPAD);
       n->v = nn_alloc<float>(n, NPAD);
       n->wb = nn_alloc<uint16_t>(n, NWB);
+      n->wbt = nn_alloc<uint16_t>(n, NWBT);
       HIPCHECK(hipMemcpy(n->w, w.data(), NPAD * sizeof(float), hipMemcpyHostToDevice));
       HIPCHECK(hipMemset(n->m, 0, NPAD * sizeof(float)));
       HIPCHECK(hipMemset(n->v, 0, NPAD * sizeof(float)));
       HIPCHECK(hipMemset(n->g, 0, NPAD * sizeof(float)));
-      k_to_bf16<<<grid_for(NPAD), 256, 0, n->stream>>>(n->w, n->wb);
+      k_to_bf16<<<grid_for(NPAD), 256, 0, n->stream>>>(n->w, n->wb, n->wbt);
       HIPCHECK(hipGetLastError());
       n->X0 = nn_alloc<uint16_t>(n, (size_t)n->cap * K0);
       for (int l = 0; l < NL; ++l) {
@@ -776,7 +822,7 @@ int fim_nn_set_params(fim_nn *n, const float *params) {
     std::vector<float> dev;
     to_device_order(params, dev);
     HIPCHECK(hipMemcpyAsync(n->w, dev.data(), NPAD * sizeof(float), hipMemcpyHostToDevice, n->stream));
-    k_to_bf16<<<grid_for(NPAD), 256, 0, n->stream>>>(n->w, n->wb);
+    k_to_bf16<<<grid_for(NPAD), 256, 0, n->stream>>>(n->w, n->wb, n->wbt);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(n->stream));
   });
