@@ -222,6 +222,11 @@ def main():
     ap.add_argument("--check-maps", type=int, default=16,
                     help="held-out maps checked against the FP64 oracle (one per host thread)")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--windows-out", default="",
+                    help="save the accounting batch's per-instance windows (.npy)")
+    ap.add_argument("--windows-in", default="",
+                    help="counter runs: replay saved per-instance windows for the accounting batch "
+                         "without the surrogate, print the roofline fields and stop")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -246,9 +251,23 @@ def main():
     kern = dict(alg_bytes=0, compute_ms=0.0, compute_launches=0, elem_updates=0, activations=0,
                 vertex_updates=0, global_iters=0, local_iters=0)
 
+    pre = dict(compute_launches=0)
+    if args.windows_in:
+        # the accounting batch alone, at the windows a full run saved: the
+        # same k_conv_band dispatches with no surrogate kernel in the process
+        acc = F.Solver(pack, n_instances=args.test, block=args.block, profile=True, unit=args.unit,
+                       local_iters=args.local_iters, window=args.window)
+        st = WS.windowed_batch(acc, test, np.load(args.windows_in))
+        acc.close()
+        print(json.dumps({"accounting_only": True, "roofline": {
+            "launches": st["compute_launches"], "launches_skipped": 0,
+            "avg_launch_us": 1e3 * st["compute_ms"] / max(st["compute_launches"], 1),
+            "alg_bytes_per_launch": st["alg_bytes"] / max(st["compute_launches"], 1),
+            "vertex_updates_per_launch": st["vertex_updates"] / max(st["compute_launches"], 1)}}), flush=True)
+        return
+
     # ---- surrogate: training maps and training (set-up) ----------------------
     nn = F.Surrogate(mesh, seed=args.seed, max_maps=args.nn_maps, max_batch=args.nn_batch)
-    pre = dict(compute_launches=0)
 
     def count(st):
         pre["compute_launches"] += st["compute_launches"]
@@ -269,7 +288,11 @@ def main():
     # ---- accounting: event-timed compute launches of one scheduled batch ----
     acc = F.Solver(pack, n_instances=args.test, block=args.block, profile=True, unit=args.unit,
                    local_iters=args.local_iters, window=args.window)
-    st, _, _ = WS.scheduled_batch(nn, acc, test, sample, window=args.window, quantile=args.quantile)
+    acc_wins = []
+    st, _, _ = WS.scheduled_batch(nn, acc, test, sample, window=args.window, quantile=args.quantile,
+                                  wins_out=acc_wins)
+    if args.windows_out:
+        np.save(args.windows_out, np.array(acc_wins, dtype=np.float64))
     for k in kern:
         kern[k] += st[k]
     acc_map0 = acc.download(inst=0)[0]

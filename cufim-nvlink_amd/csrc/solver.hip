@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -1379,14 +1380,10 @@ __device__ inline uint32_t block_sum(uint32_t v, uint32_t *slot) {
   return r;
 }
 
-// one (vertex, element) slot: src/eikonal/cuda_fim.h:1362-1411
+// one (vertex, element) slot: src/eikonal/cuda_fim.h:1362-1411.  The slot's
+// element nodes nd (local ids) and the owner's position t are given.
 __device__ inline double compat_slot(const CompatArgs &a, const SubInfo &si, const double *phi_l,
-                                     const int8_t *st_l, uint32_t k) {
-  const uint32_t le = a.n2e_le[k];
-  const int t = a.n2e_pos[k];
-  const size_t ge = (size_t)si.eoff + le;
-  const uint2 q = a.e2n[ge];
-  const uint32_t nd[4] = {q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16};
+                                     const int8_t *st_l, size_t ge, const uint32_t nd[4], int t) {
   double ph[4], g[6];
   const double lo = a.phi_start - a.fw;
   for (int j = 0; j < 4; ++j) {
@@ -1401,6 +1398,29 @@ __device__ inline double compat_slot(const CompatArgs &a, const SubInfo &si, con
     case 2: return cand_exact<2, double>(g, ph, DBL_MAX);
     default: return cand_exact<3, double>(g, ph, DBL_MAX);
   }
+}
+
+constexpr unsigned long long kDmaxBits = 0x7fefffffffffffffull;  // DBL_MAX
+
+// The candidates of every slot k0 <= k < k1 whose owner has state `want`,
+// merged into the owner's minimum (FP64 bits, LDS integer min); returns the
+// slots this thread evaluated.
+__device__ inline unsigned long long compat_slots(const CompatArgs &a, const SubInfo &si,
+                                                  const double *phi_l, const int8_t *st_l, uint32_t k0,
+                                                  uint32_t k1, int8_t want, unsigned long long *tmin) {
+  unsigned long long n = 0;
+  for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+    const int t = a.n2e_pos[k];
+    const size_t ge = (size_t)si.eoff + a.n2e_le[k];
+    const uint2 q = a.e2n[ge];
+    const uint32_t nd[4] = {q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16};
+    const uint32_t owner = nd[t];
+    if (st_l[owner] != want) continue;
+    const double v = compat_slot(a, si, phi_l, st_l, ge, nd, t);
+    atomicMin(&tmin[owner], (unsigned long long)__double_as_longlong(v));
+    ++n;
+  }
+  return n;
 }
 
 template <int BS>
@@ -1440,21 +1460,17 @@ __global__ __launch_bounds__(BS) void k_compat_compute(CompatArgs a) {
       st_l[l] = st;
     }
     uint32_t active = block_sum(cnt, misc);
+    // the submesh's (vertex, element) slots, contiguous in the copy-major CSR
+    const uint32_t k0 = a.n2e_dsp[si.coff], k1 = a.n2e_dsp[si.coff + nn];
+    unsigned long long *tmin = reinterpret_cast<unsigned long long *>(tphi);
     while (active > 0) {
       ++sweeps;
-      // a: candidates of narrow vertices
-      for (uint32_t l = tid; l < nn; l += BS) {
-        double m = DBL_MAX;
-        if (st_l[l] == FIM_NARROW) {
-          const uint32_t c = si.coff + l;
-          for (uint32_t k = a.n2e_dsp[c]; k < a.n2e_dsp[c + 1]; ++k) {
-            const double v = compat_slot(a, si, phi_l, st_l, k);
-            if (v < m) m = v;
-            ++slots;
-          }
-        }
-        tphi[l] = m;
-      }
+      // a: candidates of narrow vertices, one thread per slot; times are
+      // >= 0, so the minimum of the FP64 bit patterns as integers is the
+      // minimum of the values and does not depend on the order
+      for (uint32_t l = tid; l < nn; l += BS) tmin[l] = kDmaxBits;
+      __syncthreads();
+      slots += compat_slots(a, si, phi_l, st_l, k0, k1, FIM_NARROW, tmin);
       __syncthreads();
       // b: reduce, window test, convergence test
       for (uint32_t l = tid; l < nn; l += BS) {
@@ -1485,18 +1501,10 @@ __global__ __launch_bounds__(BS) void k_compat_compute(CompatArgs a) {
         }
       }
       __syncthreads();
-      // d: candidates of marked vertices
-      for (uint32_t l = tid; l < nn; l += BS) {
-        if (st_l[l] != FIM_NEIG) continue;
-        double m = DBL_MAX;
-        const uint32_t c = si.coff + l;
-        for (uint32_t k = a.n2e_dsp[c]; k < a.n2e_dsp[c + 1]; ++k) {
-          const double v = compat_slot(a, si, phi_l, st_l, k);
-          if (v < m) m = v;
-          ++slots;
-        }
-        tphi[l] = m;
-      }
+      // d: candidates of marked vertices, one thread per slot
+      for (uint32_t l = tid; l < nn; l += BS) tmin[l] = kDmaxBits;
+      __syncthreads();
+      slots += compat_slots(a, si, phi_l, st_l, k0, k1, FIM_NEIG, tmin);
       __syncthreads();
       // e: accept improvements
       for (uint32_t l = tid; l < nn; l += BS) {
@@ -1675,6 +1683,7 @@ struct fim_ctx {
   int grid = 0, sync_grid = 0, block = 256, graph_iters = 8;
   int band_wpe = 8;  // waves per SIMD the band kernel's registers are sized for
   bool band_pairs = false;  // band kernel evaluates (element, target) pairs
+  bool direct = false;      // launch iterations directly instead of replaying the graph
   fim_stats_t stats{};
 };
 
@@ -1924,6 +1933,13 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
       c->dev = o->device;
       HIPCHECK(hipSetDevice(c->dev));
       (void)noalloc_scratch(c->dev);  // once per device: fim_support_noalloc's scratch
+      // CUFIM_DIRECT_LAUNCH=1 launches the iterations' kernels directly instead
+      // of replaying a captured graph (the same kernel sequence): for tools
+      // that cannot trace graph launches
+      {
+        const char *dl = std::getenv("CUFIM_DIRECT_LAUNCH");
+        c->direct = dl && dl[0] == '1';
+      }
       HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       HIPCHECK(hipEventCreate(&c->ev0));
       HIPCHECK(hipEventCreate(&c->ev1));
@@ -2360,7 +2376,7 @@ uint32_t run_iterations(fim_ctx *c, int32_t max_iters, bool reset_stats) {
     c->kev.assign(2 * c->graph_iters, nullptr);
     for (auto &e : c->kev) HIPCHECK(hipEventCreate(&e));
   }
-  if (!c->o.profile && !c->gexec) build_graph(c);
+  if (!c->o.profile && !c->direct && !c->gexec) build_graph(c);
   if (reset_stats) {
     HIPCHECK(hipMemsetAsync(c->d_stats, 0, 8 * sizeof(unsigned long long), c->stream));
     c->stats.compute_ms = 0.0;
@@ -2374,8 +2390,8 @@ uint32_t run_iterations(fim_ctx *c, int32_t max_iters, bool reset_stats) {
   double kms = 0.0;
   int64_t klaunch = 0;
   while (done < max_iters) {
-    if (c->o.profile) {
-      for (int i = 0; i < c->graph_iters; ++i) enqueue_iteration(c, i, true);
+    if (c->o.profile || c->direct) {
+      for (int i = 0; i < c->graph_iters; ++i) enqueue_iteration(c, i, c->o.profile != 0);
     } else {
       HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
     }

@@ -143,6 +143,16 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, si
   }
 }
 
+// EpiLoads<E>::value: the epilogue E reads an input per output element
+template <class E, class = void>
+struct EpiLoads {
+  static constexpr bool value = false;
+};
+template <class E>
+struct EpiLoads<E, decltype(void(E::kLoads))> {
+  static constexpr bool value = E::kLoads;
+};
+
 // ---------------------------------------------------------------------------
 // The same product on 128 x 128 workgroup tiles: four waves of 64 x 64, each
 // wave 2 x 2 MFMA accumulators (32 x 32 each), so every A and B fragment read
@@ -242,16 +252,35 @@ __global__ __launch_bounds__(256) void k_gemm128(const uint16_t *__restrict__ A,
     if (st + 1 < nk) lstore(buf ^ 1);
     __syncthreads();
   }
+  if constexpr (EpiLoads<Epi>::value) {
+    // epilogues that read an input per element: all loads of one 32 x 32
+    // block first, then the stores
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < 2; ++j) {
+        uint16_t in[16];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const size_t row = r0 + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const size_t col = c0 + wc * 64 + j * 32 + (lane & 31u);
-        epi(row, col, acc[i][j][r]);
+        for (int r = 0; r < 16; ++r)
+          in[r] = epi.load(r0 + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3),
+                           c0 + wc * 64 + j * 32 + (lane & 31u));
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+          epi.store(r0 + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3),
+                    c0 + wc * 64 + j * 32 + (lane & 31u), acc[i][j][r], in[r]);
       }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const size_t row = r0 + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          const size_t col = c0 + wc * 64 + j * 32 + (lane & 31u);
+          epi(row, col, acc[i][j][r]);
+        }
+  }
 }
 
 // hidden layer forward: Z = acc + b (BF16, kept for the backward pass when Z
@@ -266,12 +295,19 @@ struct EpiFwd {
   }
 };
 
-// backward through a hidden layer: dZ_prev = (dZ W) * SiLU'(Z_prev)
+// backward through a hidden layer: dZ_prev = (dZ W) * SiLU'(Z_prev).  The
+// tile epilogue loads all its Z values (load) before it writes any dZ
+// (store), so the loads are in flight together.
 struct EpiBwd {
   const uint16_t *Zp;
   uint16_t *dZ;
+  static constexpr bool kLoads = true;
+  __device__ uint16_t load(size_t row, size_t col) const { return Zp[row * HID + col]; }
+  __device__ void store(size_t row, size_t col, float v, uint16_t z) const {
+    dZ[row * HID + col] = f2bf(v * dsilu(bf2f(z)));
+  }
   __device__ void operator()(size_t row, size_t col, float v) const {
-    dZ[row * HID + col] = f2bf(v * dsilu(bf2f(Zp[row * HID + col])));
+    store(row, col, v, load(row, col));
   }
 };
 
@@ -420,6 +456,7 @@ __global__ __launch_bounds__(256) void k_colsum(const uint16_t *dZ, uint32_t B, 
   __shared__ float part[8][HID];
   const uint32_t tid = threadIdx.x, cg = tid & 31u, rl = tid >> 5;  // column group, row lane
   float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 4
   for (uint32_t row = blockIdx.x * 8u + rl; row < B; row += gridDim.x * 8u) {
     const u16x8 v = *reinterpret_cast<const u16x8 *>(dZ + (size_t)row * HID + cg * 8u);
 #pragma unroll
@@ -435,20 +472,18 @@ __global__ __launch_bounds__(256) void k_colsum(const uint16_t *dZ, uint32_t B, 
 }
 
 // sum of the batch-split partials
-// (n a multiple of 4: 16-byte loads and stores)
+// (eight loads in flight per thread: partial k goes to accumulator k % 8,
+// and the eight are added in a fixed order, so the sum is reproducible)
 __global__ void k_reduce_parts(const float *P, uint32_t S, size_t n, float *g) {
-  const size_t n4 = n / 4;
-  const float4 *P4 = reinterpret_cast<const float4 *>(P);
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
-    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (uint32_t k = 0; k < S; ++k) {
-      const float4 v = P4[(size_t)k * n4 + i];
-      s.x += v.x;
-      s.y += v.y;
-      s.z += v.z;
-      s.w += v.w;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    float a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t k = 0;
+    for (; k + 8 <= S; k += 8) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a[j] += P[(size_t)(k + j) * n + i];
     }
-    reinterpret_cast<float4 *>(g)[i] = s;
+    for (; k < S; ++k) a[k & 7u] += P[(size_t)k * n + i];
+    g[i] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   }
 }
 
@@ -609,8 +644,8 @@ void backward(fim_nn *n, uint32_t rows) {
     else
       k_gemm<false, false, EpiPart><<<dim3(K / 64, HID / 64, S), 256, 0, n->stream>>>(dz, 1, HID, hin, K, 1, span,
                                                                                        ep);
-    k_reduce_parts<<<grid_for((size_t)HID * K / 4), 256, 0, n->stream>>>(n->parts, S, (size_t)HID * K,
-                                                                         n->g + off_w(l));
+    k_reduce_parts<<<grid_for((size_t)HID * K), 256, 0, n->stream>>>(n->parts, S, (size_t)HID * K,
+                                                                     n->g + off_w(l));
     k_colsum<<<std::min<uint32_t>(rows / 8, 512), 256, 0, n->stream>>>(dz, rows, n->g + off_b(l));
     if (l > 0) {
       // dZ_{l-1}[m][k] = sum_n dZ[m][n] W_l[n][k] * SiLU'(Z_{l-1})

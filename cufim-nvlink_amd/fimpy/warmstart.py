@@ -69,23 +69,34 @@ def scheduled_windows(tmax, window, quantile=0.75, floor=250.0):
     return np.maximum(floor, tmax / k)
 
 
-def scheduled_batch(nn: Surrogate, solver: Solver, solves, sample, window=2000.0, quantile=0.75):
-    """Surrogate-scheduled batched solve of len(solves) instances: predicted
-    latest activation per solve (one device call over the vertex sample),
-    per-instance windows, regional factors, seeds, then FIM to the fixed
-    point.  Returns (stats, seconds of prediction, seconds of the rest)."""
-    t0 = time.perf_counter()
-    tmax = nn.predict_max(solves, sample)
-    t1 = time.perf_counter()
-    wins = scheduled_windows(tmax, window, quantile)
+def windowed_batch(solver: Solver, solves, wins):
+    """Batched solve of len(solves) instances, instance i at window wins[i];
+    returns the solver stats."""
     solver.reset()
     for i, (sites, sc) in enumerate(solves):
         solver.set_scaling(sc, inst=i)
         solver.set_window(float(wins[i]), inst=i)
     solver.seed_batch([s for s, _ in solves], 0.0)
     solver.run(max_iters=1_000_000)
+    return solver.stats()
+
+
+def scheduled_batch(nn: Surrogate, solver: Solver, solves, sample, window=2000.0, quantile=0.75,
+                    wins_out=None):
+    """Surrogate-scheduled batched solve of len(solves) instances: predicted
+    latest activation per solve (one device call over the vertex sample),
+    per-instance windows, regional factors, seeds, then FIM to the fixed
+    point.  Returns (stats, seconds of prediction, seconds of the rest); the
+    windows are appended to wins_out when given."""
+    t0 = time.perf_counter()
+    tmax = nn.predict_max(solves, sample)
+    t1 = time.perf_counter()
+    wins = scheduled_windows(tmax, window, quantile)
+    if wins_out is not None:
+        wins_out.extend(float(w) for w in wins)
+    st = windowed_batch(solver, solves, wins)
     t2 = time.perf_counter()
-    return solver.stats(), t1 - t0, t2 - t1
+    return st, t1 - t0, t2 - t1
 
 
 def cold_batch(solver, solves, window=None):
