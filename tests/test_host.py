@@ -481,3 +481,26 @@ def test_integration_adapter_runs_from_the_reference_side():
     assert "num_stims: 1" in out  # the reference's plan parser ran
     assert r.returncode == 0 or "hipSetDevice" in out, out
 
+
+
+def test_scheduled_windows_and_vertex_sample():
+    """per-instance windows: the batch runs quantile(tmax) / window
+    iterations and solve j gets tmax_j / K (never below the floor); the
+    vertex sample holds the bounding-box corners"""
+    from fimpy import warmstart as WS
+    tmax = np.array([100e3, 200e3, 300e3, 400e3, 500e3])
+    w = WS.scheduled_windows(tmax, 2000.0, quantile=0.5)
+    K = 300e3 / 2000.0
+    assert np.allclose(w, tmax / K)
+    assert np.isclose(w[2], 2000.0)  # the median solve keeps the base window
+    w = WS.scheduled_windows(np.array([1e3, 400e3]), 2000.0, quantile=1.0, floor=250.0)
+    assert w[0] == 250.0 and np.isclose(w[1], 2000.0)
+    m = F.Mesh.gen_box(seed=1, **dict(cases.CUBE, nx=4, ny=4, nz=4))
+    sample = WS.window_sample(m, k=10, seed=3)
+    xyz, _, _ = m.arrays()
+    lo, hi = xyz.min(axis=0), xyz.max(axis=0)
+    for c in ((lo[0], lo[1], lo[2]), (hi[0], hi[1], hi[2]), (lo[0], hi[1], lo[2])):
+        nearest = int(np.argmin(np.sum((xyz - np.array(c)) ** 2, axis=1)))
+        assert nearest in sample
+    assert len(np.unique(sample)) == len(sample) <= 18
+    assert np.array_equal(sample, WS.window_sample(m, k=10, seed=3))  # seeded
