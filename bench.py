@@ -90,7 +90,8 @@ def kernel_source_digest():
 
 def config_key(args):
     return (f"held_out={args.test} max_nn={args.max_nn} block={args.block} unit={args.unit} "
-            f"local_iters={args.local_iters} window={float(args.window)} quantile={args.quantile}")
+            f"local_iters={args.local_iters} window={float(args.window)} quantile={args.quantile} "
+            f"sample={args.sample}")
 
 
 def profile_figures(args, name):
@@ -198,6 +199,9 @@ def main():
     ap.add_argument("--quantile", type=float, default=0.75,
                     help="scheduled windows: the batch runs quantile(predicted latest time) / window "
                          "iterations")
+    ap.add_argument("--sample", type=int, default=1000,
+                    help="random vertices (besides the 8 bounding-box corners) the latest activation "
+                         "is predicted at")
     ap.add_argument("--nn-maps", type=int, default=4096)
     ap.add_argument("--nn-steps", type=int, default=4000)
     ap.add_argument("--nn-batch", type=int, default=65536)
@@ -283,7 +287,7 @@ def main():
         done += k
         log(f"[bench] surrogate training {done}/{args.nn_steps} steps, loss {loss:.3e}")
     t_train = time.perf_counter() - t0
-    sample = WS.window_sample(mesh, seed=args.seed)
+    sample = WS.window_sample(mesh, k=args.sample, seed=args.seed)
 
     # ---- accounting: event-timed compute launches of one scheduled batch ----
     acc = F.Solver(pack, n_instances=args.test, block=args.block, profile=True, unit=args.unit,
@@ -475,6 +479,7 @@ def main():
             "local_iters": args.local_iters,
             "window_us": args.window,
             "quantile": args.quantile,
+            "sample_vertices": len(sample),
         },
         "max_abs_dT_us": max_dt,
         "tolerance_us": 100.0,

@@ -9,11 +9,16 @@
 // Device layout of the FP32 master parameters (the first layer is padded to
 // 64 inputs, the padding weights stay zero):
 //   W0[256 x 64] b0[256]  (W_l[256 x 256] b_l[256]) x 5  w_out[256] b_out
-// Activations of a batch of B rows (B a multiple of 64) are row-major BF16:
-//   X0[B x 64], Z_l and H_l = SiLU(Z_l) [B x 256] for l = 0..5.
-// Backward: dZ_5 from the head, then per layer dW_l = dZ_l^T H_{l-1} (split
-// over the batch into partial sums, reduced), db_l = column sums of dZ_l,
-// dZ_{l-1} = (dZ_l W_l) * SiLU'(Z_{l-1}).
+// A training step is three kernels:
+//   k_mlp_chain<true>: per tile of 64 rows, the six forward layers with the
+//     activations kept in LDS, the output layer and squared error, and the
+//     five backward-data layers dZ_{l-1} = (dZ_l W_l) * SiLU'(Z_{l-1});
+//     Z_0..Z_4 and dZ_0..dZ_5 leave the tile for the weight gradients;
+//   k_mlp_wgrad: dW_l = dZ_l^T H_{l-1} and db_l = column sums of dZ_l as
+//     partial sums over spans of rows (H = SiLU(Z) recomputed from Z);
+//   k_reduce_parts + k_adam: the sum of the partials, the Adam update and
+//     the BF16 weight copies.
+// Prediction is k_mlp_chain<false> (forward and output layer only).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,11 +53,11 @@ constexpr size_t off_b(int l) { return off_w(l) + (l == 0 ? W0_N : WL_N); }
 constexpr size_t OFF_WO = off_w(NL);  // = off_b(5) + HID
 constexpr size_t OFF_BO = OFF_WO + HID;
 constexpr size_t NPAD = OFF_BO + 1;
-// BF16 weight copies
+// BF16 weight copies in MFMA fragment order (below): wb holds W_l as the
+// operand of the forward products (l = 0..5), wbt W_l^T as the operand of the
+// backward-data products (l = 1..5)
 constexpr size_t offb_w(int l) { return l == 0 ? 0 : W0_N + (size_t)(l - 1) * WL_N; }
 constexpr size_t NWB = W0_N + (size_t)(NL - 1) * WL_N;
-// BF16 copies of the hidden layers transposed (WT_l[k][n] = W_l[n][k], l >= 1):
-// the backward pass reads them contiguous in its K dimension
 constexpr size_t offbt(int l) { return (size_t)(l - 1) * WL_N; }
 constexpr size_t NWBT = (size_t)(NL - 1) * WL_N;
 static_assert(off_w(NL) == off_b(NL - 1) + HID, "layout");
@@ -64,12 +69,17 @@ typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// float -> bf16, round to nearest even (finite inputs)
+// float -> bf16, round to nearest even (finite inputs): the hardware
+// conversion on the device, the same rounding written out on the host
 __host__ __device__ inline uint16_t f2bf(float f) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
+#else
   uint32_t u;
   std::memcpy(&u, &f, 4);
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
+#endif
 }
 __host__ __device__ inline float bf2f(uint16_t h) {
   const uint32_t u = (uint32_t)h << 16;
@@ -77,7 +87,9 @@ __host__ __device__ inline float bf2f(uint16_t h) {
   std::memcpy(&f, &u, 4);
   return f;
 }
-__device__ inline float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// logistic function with the hardware exponential and reciprocal (about 1
+// ulp each; the results are rounded to BF16 afterwards)
+__device__ inline float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ inline float silu(float z) { return z * sigm(z); }
 __device__ inline float dsilu(float z) {
   const float s = sigm(z);
@@ -85,240 +97,415 @@ __device__ inline float dsilu(float z) {
 }
 
 // ---------------------------------------------------------------------------
-// C[R x Cc] = A(R x K) B(K x Cc) over the K range of blockIdx.z, BF16 in,
-// FP32 accumulate.  A(r, k) = A[r*sar + k*sak], B(k, c) = B[k*sbk + c*sbc];
-// AK: A contiguous in k (sak == 1), else in r (sar == 1); BK likewise for B.
-// Workgroup tile 64 x 64, four waves of 32 x 32 (one MFMA accumulator),
-// K staged through LDS 32 at a time.  R, Cc multiples of 64; the K span a
-// multiple of 32.  MFMA 32x32x16 operands: lane l holds row (col) l % 32,
-// k = 8 * (l / 32) + 0..7 of a 16-deep step; the accumulator element r of
-// lane l is row 8 * (r / 4) + 4 * (l / 32) + r % 4, col l % 32.
+// Layouts of the fused kernels.
+//
+// k-step layout of a matrix M[rows][C] (activations Z, dZ and the inputs X0):
+// rows in groups of 16, each group stored column by column, so the 16 values
+// of one column in one group are 32 contiguous bytes:
+//   M(row, c) at ((row / 16) * C + c) * 16 + row % 16.
+// An MFMA operand fragment whose contraction index is the row (weight
+// gradients) is one 16-byte load in this layout.
+//
+// Fragment order of an operand A[M][K] of v_mfma_f32_32x32x16_bf16: lane l of
+// a 32 x 16 fragment holds A(32 mb + l % 32, 16 ks + 8 (l / 32) + 0..7), and
+// fragment (mb, ks) is the 1 KB block (mb * K / 16 + ks) -- one coalesced
+// 16-byte load per lane.  wb holds the hidden-layer weights W_l[n][k] with
+// m = n (forward products), wbt the layers l >= 1 with m = k (backward-data
+// products).
+//
+// The accumulator element r of lane l of a 32 x 32 product is
+// (row 8 (r / 4) + 4 (l / 32) + r % 4, col l % 32).
 // ---------------------------------------------------------------------------
-template <bool AK, bool BK, class Epi>
-__global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, size_t sar, size_t sak,
-                                              const uint16_t *__restrict__ B, size_t sbk, size_t sbc,
-                                              uint32_t kspan, Epi epi) {
-  __shared__ __align__(16) uint16_t As[64][40];  // [row][k], rows padded to 80 B
-  __shared__ __align__(16) uint16_t Bs[64][40];  // [col][k]
-  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const uint32_t wr = w & 1u, wc = w >> 1;
-  const size_t r0 = (size_t)blockIdx.y * 64, c0 = (size_t)blockIdx.x * 64;
-  const size_t kb = (size_t)blockIdx.z * kspan, ke = kb + kspan;
-  f32x16 acc;
-  for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-  for (size_t k0 = kb; k0 < ke; k0 += 32) {
-    if (AK) {
-      const uint32_t r = tid >> 2, kq = (tid & 3u) * 8u;
-      *reinterpret_cast<u16x8 *>(&As[r][kq]) =
-          *reinterpret_cast<const u16x8 *>(A + (r0 + r) * sar + k0 + kq);
-    } else {
-      const uint32_t k = tid >> 3, rq = (tid & 7u) * 8u;
-      const u16x8 v = *reinterpret_cast<const u16x8 *>(A + (k0 + k) * sak + r0 + rq);
-      for (int i = 0; i < 8; ++i) As[rq + i][k] = v[i];
-    }
-    if (BK) {
-      const uint32_t c = tid >> 2, kq = (tid & 3u) * 8u;
-      *reinterpret_cast<u16x8 *>(&Bs[c][kq]) =
-          *reinterpret_cast<const u16x8 *>(B + (c0 + c) * sbc + k0 + kq);
-    } else {
-      const uint32_t k = tid >> 3, cq = (tid & 7u) * 8u;
-      const u16x8 v = *reinterpret_cast<const u16x8 *>(B + (k0 + k) * sbk + c0 + cq);
-      for (int i = 0; i < 8; ++i) Bs[cq + i][k] = v[i];
-    }
-    __syncthreads();
-#pragma unroll
-    for (uint32_t kk = 0; kk < 32; kk += 16) {
-      const u16x8 a = *reinterpret_cast<const u16x8 *>(&As[wr * 32 + (lane & 31u)][kk + (lane >> 5) * 8]);
-      const u16x8 b = *reinterpret_cast<const u16x8 *>(&Bs[wc * 32 + (lane & 31u)][kk + (lane >> 5) * 8]);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
-                                                   __builtin_bit_cast(bf16x8, b), acc, 0, 0, 0);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const size_t row = r0 + wr * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-    const size_t col = c0 + wc * 32 + (lane & 31u);
-    epi(row, col, acc[r]);
-  }
+__host__ __device__ inline size_t ks_off(size_t row, uint32_t c, uint32_t C) {
+  return (((row >> 4) * C + c) << 4) + (row & 15u);
+}
+__host__ __device__ constexpr size_t frag_off(uint32_t m, uint32_t k, uint32_t K) {
+  return ((size_t)((m >> 5) * (K >> 4) + (k >> 4)) * 64u + (m & 31u) + 32u * ((k >> 3) & 1u)) * 8u + (k & 7u);
 }
 
-// EpiLoads<E>::value: the epilogue E reads an input per output element
-template <class E, class = void>
-struct EpiLoads {
-  static constexpr bool value = false;
-};
-template <class E>
-struct EpiLoads<E, decltype(void(E::kLoads))> {
-  static constexpr bool value = E::kLoads;
-};
+__device__ inline f32x16 mfma(const u16x8 a, const u16x8 b, const f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                 0, 0, 0);
+}
 
-// ---------------------------------------------------------------------------
-// The same product on 128 x 128 workgroup tiles: four waves of 64 x 64, each
-// wave 2 x 2 MFMA accumulators (32 x 32 each), so every A and B fragment read
-// from LDS feeds two MFMAs.  K advances 32 at a time through two LDS buffers:
-// the next step's global loads are issued into registers before the current
-// step's MFMAs and stored into the other buffer after them, one barrier per
-// step.  R, Cc multiples of 128; the K span a multiple of 32.
-// ---------------------------------------------------------------------------
-template <bool AK, bool BK, class Epi>
-__global__ __launch_bounds__(256) void k_gemm128(const uint16_t *__restrict__ A, size_t sar, size_t sak,
-                                                 const uint16_t *__restrict__ B, size_t sbk, size_t sbc,
-                                                 uint32_t kspan, Epi epi) {
-  constexpr int LD = 40;  // LDS row pitch in BF16 values (80 B)
-  __shared__ __align__(16) uint16_t As[2][128][LD];  // [buffer][row][k]
-  __shared__ __align__(16) uint16_t Bs[2][128][LD];  // [buffer][col][k]
-  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const uint32_t wr = w & 1u, wc = w >> 1;
-  const size_t r0 = (size_t)blockIdx.y * 128, c0 = (size_t)blockIdx.x * 128;
-  const size_t kb = (size_t)blockIdx.z * kspan;
-  const uint32_t nk = kspan / 32;
-  u16x8 ra[2], rb[2];
-  auto gload = [&](size_t k0) {
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      if (AK) {
-        const uint32_t r = (tid >> 2) + 64u * j, kq = (tid & 3u) * 8u;
-        ra[j] = *reinterpret_cast<const u16x8 *>(A + (r0 + r) * sar + k0 + kq);
-      } else {  // rows k0 + 2 (tid >> 4) + j: two neighbouring k per thread
-        const uint32_t k = 2u * (tid >> 4) + j, rq = (tid & 15u) * 8u;
-        ra[j] = *reinterpret_cast<const u16x8 *>(A + (k0 + k) * sak + r0 + rq);
-      }
-      if (BK) {
-        const uint32_t c = (tid >> 2) + 64u * j, kq = (tid & 3u) * 8u;
-        rb[j] = *reinterpret_cast<const u16x8 *>(B + (c0 + c) * sbc + k0 + kq);
-      } else {
-        const uint32_t k = 2u * (tid >> 4) + j, cq = (tid & 15u) * 8u;
-        rb[j] = *reinterpret_cast<const u16x8 *>(B + (k0 + k) * sbk + c0 + cq);
-      }
-    }
-  };
-  // transposed staging writes the two neighbouring k of a row as one
-  // 32-bit word (k is even, rows are 80 B apart, so the word is aligned)
-  auto lstore = [&](int buf) {
-    if (AK) {
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const uint32_t r = (tid >> 2) + 64u * j, kq = (tid & 3u) * 8u;
-        *reinterpret_cast<u16x8 *>(&As[buf][r][kq]) = ra[j];
-      }
-    } else {
-      const uint32_t k = 2u * (tid >> 4), rq = (tid & 15u) * 8u;
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        *reinterpret_cast<uint32_t *>(&As[buf][rq + i][k]) = (uint32_t)ra[0][i] | ((uint32_t)ra[1][i] << 16);
-    }
-    if (BK) {
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const uint32_t c = (tid >> 2) + 64u * j, kq = (tid & 3u) * 8u;
-        *reinterpret_cast<u16x8 *>(&Bs[buf][c][kq]) = rb[j];
-      }
-    } else {
-      const uint32_t k = 2u * (tid >> 4), cq = (tid & 15u) * 8u;
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        *reinterpret_cast<uint32_t *>(&Bs[buf][cq + i][k]) = (uint32_t)rb[0][i] | ((uint32_t)rb[1][i] << 16);
-    }
-  };
-  f32x16 acc[2][2];
+constexpr int RG = 1;         // row groups of 64 per tile of the fused chain kernel
+constexpr int TR = 64 * RG;   // rows per tile
+constexpr int NT = 256 * RG;  // threads per workgroup: four waves per row group
+constexpr int HP = HID + 8;   // LDS row pitch of the tile (BF16 values)
+// LDS of the chain kernel: the tile, w_out, dy per row, loss and
+// bias-gradient sums
+constexpr size_t CHAIN_LDS = (size_t)TR * HP * 2 + (HID + TR + 2) * 4;
+
+// One layer's product for the tile: acc[i][j] (rows 64 rg + 32 i, columns
+// 64 fg + 32 j) = T(rows, 0..K) B, with T the tile in LDS (row-major, pitch
+// HP) and B given by fragments (fragment (mb, ks) of B^T at bfr).  The next
+// step's weight fragments are loaded before the current step's products.
+template <int K>
+__device__ inline void tile_product(const uint16_t *Ts, const uint16_t *__restrict__ bfr, uint32_t rg, uint32_t fg,
+                                    uint32_t lane, f32x16 (&acc)[2][2]) {
+  constexpr int NS = K / 16;
+  const uint32_t l32 = lane & 31u, lh = lane >> 5;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-  gload(kb);
-  lstore(0);
+  const u16x8 *bp0 = reinterpret_cast<const u16x8 *>(bfr) + (size_t)(2 * fg) * NS * 64 + lane;
+  const u16x8 *bp1 = bp0 + NS * 64;
+  const uint16_t *ta0 = Ts + (64 * rg + l32) * HP + 8 * lh;
+  const uint16_t *ta1 = ta0 + 32 * HP;
+  // weight fragments two steps ahead, tile fragments one step ahead
+  u16x8 b[2][2], t[2][2];
+  b[0][0] = bp0[0];
+  b[0][1] = bp1[0];
+  b[1][0] = bp0[64];
+  b[1][1] = bp1[64];
+  t[0][0] = *reinterpret_cast<const u16x8 *>(ta0);
+  t[0][1] = *reinterpret_cast<const u16x8 *>(ta1);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int c = s & 1;
+    if (s + 1 < NS) {
+      t[c ^ 1][0] = *reinterpret_cast<const u16x8 *>(ta0 + 16 * (s + 1));
+      t[c ^ 1][1] = *reinterpret_cast<const u16x8 *>(ta1 + 16 * (s + 1));
+    }
+    acc[0][0] = mfma(t[c][0], b[c][0], acc[0][0]);
+    acc[0][1] = mfma(t[c][0], b[c][1], acc[0][1]);
+    acc[1][0] = mfma(t[c][1], b[c][0], acc[1][0]);
+    acc[1][1] = mfma(t[c][1], b[c][1], acc[1][1]);
+    if (s + 2 < NS) {
+      b[c][0] = bp0[(s + 2) * 64];
+      b[c][1] = bp1[(s + 2) * 64];
+    }
+    // keep each step's loads in place (no hoisting across steps): the
+    // fragments stay two steps ahead without holding the whole layer
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+struct ChainArgs {
+  const uint16_t *X0;        // inputs, k-step layout, K0 columns
+  const uint16_t *wb, *wbt;  // fragment-ordered BF16 weights
+  const float *w;            // FP32 masters (biases, output layer)
+  const float *tgt;          // training: targets / t_scale
+  uint16_t *Z[NL - 1];       // training: Z_0..Z_4, k-step layout
+  uint16_t *dZ[NL];          // training: dZ_0..dZ_5, k-step layout
+  float *g, *loss;           // training: output-layer gradient (accumulated), loss sum
+  float *out;                // inference: predictions
+  uint32_t nreal;            // rows with data; the rest of the last tile is padding
+  float inv_b, t_scale, delta;
+};
+
+// The whole network for one tile of 128 rows, activations in LDS between
+// layers.  Eight waves: rows 64 rg .. +63 (rg = wave / 4), columns 64 fg ..
+// +63 (fg = wave % 4) of every product.
+//   forward: Z_l = H_{l-1} W_l^T + b_l, H_l = SiLU(Z_l) (BF16 in LDS; Z_l to
+//     HBM in the k-step layout when training);
+//   output: y = H_5 . w_out + b_out (inference: t = y t_scale + delta);
+//   training: e = y - t, dZ_5 = (2 e / B) w_out SiLU'(Z_5), the output-layer
+//     gradient, then dZ_{l-1} = (dZ_l W_l) * SiLU'(Z_{l-1}) for l = 5..1 with
+//     dZ_l in LDS; every dZ_l goes to HBM in the k-step layout.
+// Rounding to BF16 where the separate-kernel formulation stores values:
+// Z, H and dZ; SiLU'(Z) of the stored Z.
+template <bool TRAIN>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_mlp_chain(ChainArgs a) {
+  extern __shared__ __align__(16) unsigned char chain_lds[];
+  uint16_t *Ts = reinterpret_cast<uint16_t *>(chain_lds);
+  float *wos = reinterpret_cast<float *>(chain_lds + TR * HP * 2);  // w_out
+  float *dys = wos + HID;
+  float *misc = dys + TR;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t rg = wave >> 2, fg = wave & 3u, l32 = lane & 31u, lh = lane >> 5;
+  const size_t r0 = (size_t)blockIdx.x * TR;
+  // k-step offset, within the tile's rows, of the four accumulator values
+  // (rows 64 rg + 32 i + 8 q + 4 lh + 0..3, column 64 fg + 32 j + l32) of
+  // block (i, j), group q: ((row / 16) HID + column) 16 + row % 16
+  const uint32_t ks_lane = ((4 * rg * HID + 64 * fg + l32) << 4) + 4 * lh;
+  auto ks_acc = [&](int i, int j, int q) -> uint32_t {
+    return ks_lane + i * 32 * HID + (q >> 1) * 16 * HID + j * 32 * 16 + (q & 1) * 8;
+  };
+  if (tid < 2) misc[tid] = 0.0f;
+  for (uint32_t c = tid; c < HID; c += NT) wos[c] = a.w[OFF_WO + c];
+  // inputs: 16-byte pieces of the k-step layout (8 rows of one column)
+  for (uint32_t p = tid; p < TR * K0 / 8; p += NT) {
+    const uint32_t kb = p / (2 * K0), c = (p >> 1) % K0, hf = p & 1u;
+    const u16x8 v = *reinterpret_cast<const u16x8 *>(a.X0 + ((((r0 >> 4) + kb) * K0 + c) << 4) + 8 * hf);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) Ts[(kb * 16 + 8 * hf + j) * HP + c] = v[j];
+  }
   __syncthreads();
-  for (uint32_t st = 0; st < nk; ++st) {
-    const int buf = st & 1;
-    if (st + 1 < nk) gload(kb + 32 * (size_t)(st + 1));
+  f32x16 acc[2][2];
+#pragma unroll 1
+  for (int l = 0; l < NL; ++l) {
+    if (l == 0)
+      tile_product<K0>(Ts, a.wb + offb_w(0), rg, fg, lane, acc);
+    else
+      tile_product<HID>(Ts, a.wb + offb_w(l), rg, fg, lane, acc);
+    __syncthreads();  // every wave has read the layer's input
 #pragma unroll
-    for (uint32_t kk = 0; kk < 32; kk += 16) {
-      u16x8 a[2], b[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        a[i] = *reinterpret_cast<const u16x8 *>(&As[buf][wr * 64 + i * 32 + (lane & 31u)][kk + (lane >> 5) * 8]);
-        b[i] = *reinterpret_cast<const u16x8 *>(&Bs[buf][wc * 64 + i * 32 + (lane & 31u)][kk + (lane >> 5) * 8]);
-      }
+    for (int j = 0; j < 2; ++j) {
+      const uint32_t f = 64 * fg + 32 * j + l32;
+      const float bias = a.w[off_b(l) + f];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a[i]),
-                                                             __builtin_bit_cast(bf16x8, b[j]), acc[i][j], 0, 0, 0);
+        for (int r = 0; r < 16; ++r) acc[i][j][r] += bias;
     }
-    if (st + 1 < nk) lstore(buf ^ 1);
-    __syncthreads();
-  }
-  if constexpr (EpiLoads<Epi>::value) {
-    // epilogues that read an input per element: all loads of one 32 x 32
-    // block first, then the stores
+    uint16_t *zl = TRAIN && l < NL - 1 ? a.Z[l] + r0 * HID : nullptr;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        uint16_t in[16];
+        const uint32_t f = 64 * fg + 32 * j + l32;
 #pragma unroll
-        for (int r = 0; r < 16; ++r)
-          in[r] = epi.load(r0 + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3),
-                           c0 + wc * 64 + j * 32 + (lane & 31u));
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t R = 64 * rg + 32 * i + 8 * q + 4 * lh;
+          if (zl) {
+            u16x4 zv;
 #pragma unroll
-        for (int r = 0; r < 16; ++r)
-          epi.store(r0 + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3),
-                    c0 + wc * 64 + j * 32 + (lane & 31u), acc[i][j][r], in[r]);
+            for (int c = 0; c < 4; ++c) zv[c] = f2bf(acc[i][j][4 * q + c]);
+            *reinterpret_cast<u16x4 *>(zl + ks_acc(i, j, q)) = zv;
+          }
+#pragma unroll
+          for (int c = 0; c < 4; ++c) Ts[(R + c) * HP + f] = f2bf(silu(acc[i][j][4 * q + c]));
+        }
       }
-  } else {
+    __syncthreads();
+  }
+  // output layer: H_5 in LDS, Z_5 (FP32) in acc.  Four threads per row, 64
+  // columns each.
+  {
+    const uint32_t row = tid >> 2, c0 = (tid & 3u) * 64;
+    float p = 0.0f;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const u16x8 h = *reinterpret_cast<const u16x8 *>(Ts + row * HP + c0 + 8 * u);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) p += bf2f(h[k]) * wos[c0 + 8 * u + k];
+    }
+    p += __shfl_xor(p, 1);
+    p += __shfl_xor(p, 2);
+    const float y = p + a.w[OFF_BO];
+    const size_t grow = r0 + row;
+    if (!TRAIN) {
+      if ((tid & 3u) == 0 && grow < a.nreal) a.out[grow] = y * a.t_scale + a.delta;
+      return;
+    }
+    const float e = grow < a.nreal ? y - a.tgt[grow] : 0.0f;
+    const float dy = 2.0f * e * a.inv_b;
+    if ((tid & 3u) == 0) dys[row] = dy;
+    float ls = (tid & 3u) == 0 ? e * e : 0.0f, gb = (tid & 3u) == 0 ? dy : 0.0f;
+    for (int off = 32; off > 0; off >>= 1) {
+      ls += __shfl_xor(ls, off);
+      gb += __shfl_xor(gb, off);
+    }
+    if (lane == 0) {
+      atomicAdd(&misc[0], ls);
+      atomicAdd(&misc[1], gb);
+    }
+  }
+  __syncthreads();
+  // output-layer gradient sum_rows dy H_5 (one column per thread)
+  if (tid < HID) {
+    float gw = 0.0f;
+    for (uint32_t row = 0; row < TR; ++row) gw += dys[row] * bf2f(Ts[row * HP + tid]);
+    atomicAdd(&a.g[OFF_WO + tid], gw);
+  }
+  if (tid == 0) {
+    atomicAdd(a.loss, misc[0]);
+    atomicAdd(&a.g[OFF_BO], misc[1]);
+  }
+  __syncthreads();  // H_5 is read; dZ_5 takes its place
+  // dZ_5 = dy w_out SiLU'(Z_5)
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const uint32_t f = 64 * fg + 32 * j + l32;
+    const float wo = wos[f];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t R = 64 * rg + 32 * i + 8 * q + 4 * lh;
+        u16x4 dv;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          dv[c] = f2bf(dys[R + c] * wo * dsilu(bf2f(f2bf(acc[i][j][4 * q + c]))));
+          Ts[(R + c) * HP + f] = dv[c];
+        }
+        *reinterpret_cast<u16x4 *>(a.dZ[NL - 1] + r0 * HID + ks_acc(i, j, q)) = dv;
+      }
+  }
+  __syncthreads();
+  // backward through the hidden layers
+#pragma unroll 1
+  for (int l = NL - 1; l >= 1; --l) {
+    tile_product<HID>(Ts, a.wbt + offbt(l), rg, fg, lane, acc);
+    __syncthreads();  // every wave has read dZ_l
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      u16x4 zp[2][4];  // Z_{l-1} of this row block, loaded before any is used
+#pragma unroll
       for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const size_t row = r0 + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-          const size_t col = c0 + wc * 64 + j * 32 + (lane & 31u);
-          epi(row, col, acc[i][j][r]);
+        for (int q = 0; q < 4; ++q)
+          zp[j][q] = *reinterpret_cast<const u16x4 *>(a.Z[l - 1] + r0 * HID + ks_acc(i, j, q));
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const uint32_t k = 64 * fg + 32 * j + l32;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t R = 64 * rg + 32 * i + 8 * q + 4 * lh;
+          u16x4 dv;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            dv[c] = f2bf(acc[i][j][4 * q + c] * dsilu(bf2f(zp[j][q][c])));
+            Ts[(R + c) * HP + k] = dv[c];
+          }
+          *reinterpret_cast<u16x4 *>(a.dZ[l - 1] + r0 * HID + ks_acc(i, j, q)) = dv;
         }
+      }
+    }
+    __syncthreads();
   }
 }
 
-// hidden layer forward: Z = acc + b (BF16, kept for the backward pass when Z
-// is given), H = SiLU(Z)
-struct EpiFwd {
-  const float *bias;
-  uint16_t *Z, *H;
-  __device__ void operator()(size_t row, size_t col, float v) const {
-    const float z = v + bias[col];
-    if (Z) Z[row * HID + col] = f2bf(z);
-    H[row * HID + col] = f2bf(silu(z));
-  }
+// ---------------------------------------------------------------------------
+// Weight gradients dW_l = dZ_l^T H_{l-1} and db_l = column sums of dZ_l over
+// one span of rows per workgroup (blockIdx.x = span, blockIdx.y = layer),
+// written as that span's partial sum (P[span][off_w(l) ..], the master
+// layout).  H_{l-1} = SiLU(Z_{l-1}) for l >= 1, X0 for l = 0.  Operands are
+// staged 32 rows at a time into LDS in their k-step layout (SiLU applied
+// once per element); every fragment is then one 16-byte LDS read.  Eight
+// waves tile the 256 x K output: 4 x 2 waves of 64 x 128 (K = 256) or 8 x 1
+// waves of 32 x 64 (K = 64).
+// ---------------------------------------------------------------------------
+struct WGradArgs {
+  const uint16_t *dZ[NL];   // k-step layout, HID columns
+  const uint16_t *Hin[NL];  // l = 0: X0 (K0 columns); l >= 1: Z_{l-1} (HID columns)
+  float *P;                 // partial sums, OFF_WO floats per span
+  uint32_t steps;           // 32-row steps of the batch; span s takes steps
+                            // [s steps / S, (s + 1) steps / S), S = gridDim.x
 };
 
-// backward through a hidden layer: dZ_prev = (dZ W) * SiLU'(Z_prev).  The
-// tile epilogue loads all its Z values (load) before it writes any dZ
-// (store), so the loads are in flight together.
-struct EpiBwd {
-  const uint16_t *Zp;
-  uint16_t *dZ;
-  static constexpr bool kLoads = true;
-  __device__ uint16_t load(size_t row, size_t col) const { return Zp[row * HID + col]; }
-  __device__ void store(size_t row, size_t col, float v, uint16_t z) const {
-    dZ[row * HID + col] = f2bf(v * dsilu(bf2f(z)));
-  }
-  __device__ void operator()(size_t row, size_t col, float v) const {
-    store(row, col, v, load(row, col));
-  }
-};
+constexpr int WG_LDS = 2 * 2 * 2 * HID * 16;  // BF16 values: 2 buffers x (dZ + H) x 32 rows x 256
 
-// weight gradient: partial sum of one batch split
-struct EpiPart {
-  float *P;
-  uint32_t ld;
-  __device__ void operator()(size_t row, size_t col, float v) const {
-    P[(size_t)blockIdx.z * HID * ld + row * ld + col] = v;
+template <int K>
+__device__ inline void wgrad_body(const WGradArgs &a, uint32_t l, uint16_t *lds) {
+  constexpr int WN = K == HID ? 4 : 8, WK = 8 / WN;
+  constexpr int BN = HID / WN / 32, BK = K / WK / 32;
+  constexpr uint32_t BUF = 2 * 2 * HID * 16;  // one buffer: dZ (2 x HID x 16) then H (2 x K x 16)
+  constexpr uint32_t HPC = 2 * K * 16 / 8;    // 16-byte pieces of H per step
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t wn = wave / WK, wk = wave % WK, l32 = lane & 31u, lh = lane >> 5;
+  const uint32_t s0 = (uint32_t)((uint64_t)blockIdx.x * a.steps / gridDim.x);
+  const uint32_t s1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * a.steps / gridDim.x);
+  const uint16_t *dz = a.dZ[l], *hz = a.Hin[l];  // k-step blocks of consecutive rows are contiguous
+  // two register slots of staged global loads: step st + 2 is loaded while
+  // step st is multiplied and step st + 1 is stored to LDS
+  u16x8 rz[2][2], rh[2][2];
+  float csum = 0.0f;  // column sum of dZ for column tid / 2
+  auto gload = [&](int slot, uint32_t st) {
+    const u16x8 *sz = reinterpret_cast<const u16x8 *>(dz + (size_t)st * 32 * HID);
+    const u16x8 *sh = reinterpret_cast<const u16x8 *>(hz + (size_t)st * 32 * K);
+    rz[slot][0] = sz[tid];
+    rz[slot][1] = sz[tid + 512];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const uint32_t p = tid + 512u * u;
+      if (p < HPC) rh[slot][u] = sh[p];
+    }
+  };
+  auto lstore = [&](int slot, uint32_t buf) {
+    u16x8 *bz = reinterpret_cast<u16x8 *>(lds + buf * BUF);
+    u16x8 *bh = reinterpret_cast<u16x8 *>(lds + buf * BUF + 2 * HID * 16);
+    bz[tid] = rz[slot][0];
+    bz[tid + 512] = rz[slot][1];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) csum += bf2f(rz[slot][u][j]);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const uint32_t p = tid + 512u * u;
+      if (p < HPC) {
+        u16x8 v = rh[slot][u];
+        if (K == HID)
+#pragma unroll
+          for (int j = 0; j < 8; ++j) v[j] = f2bf(silu(bf2f(v[j])));
+        bh[p] = v;
+      }
+    }
+  };
+  f32x16 acc[BN][BK];
+#pragma unroll
+  for (int i = 0; i < BN; ++i)
+#pragma unroll
+    for (int j = 0; j < BK; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+  auto product = [&](uint32_t buf) {
+    const uint16_t *bz = lds + buf * BUF, *bh = lds + buf * BUF + 2 * HID * 16;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {  // two 16-row k-steps
+      u16x8 fa[BN], fb[BK];
+#pragma unroll
+      for (int i = 0; i < BN; ++i)
+        fa[i] = *reinterpret_cast<const u16x8 *>(bz + ((u * HID + wn * BN * 32 + 32 * i + l32) << 4) + 8 * lh);
+#pragma unroll
+      for (int j = 0; j < BK; ++j)
+        fb[j] = *reinterpret_cast<const u16x8 *>(bh + ((u * K + wk * BK * 32 + 32 * j + l32) << 4) + 8 * lh);
+#pragma unroll
+      for (int i = 0; i < BN; ++i)
+#pragma unroll
+        for (int j = 0; j < BK; ++j) acc[i][j] = mfma(fa[i], fb[j], acc[i][j]);
+    }
+  };
+  if (s0 < s1) {
+    gload(0, s0);
+    lstore(0, 0);
+    if (s0 + 1 < s1) gload(1, s0 + 1);
+    if (s0 + 2 < s1) gload(0, s0 + 2);
+    __syncthreads();
+    for (uint32_t st = s0; st < s1; st += 2) {
+      product(0);
+      if (st + 1 < s1) {
+        lstore(1, 1);
+        if (st + 3 < s1) gload(1, st + 3);
+      }
+      __syncthreads();
+      if (st + 1 >= s1) break;
+      product(1);
+      if (st + 2 < s1) {
+        lstore(0, 0);
+        if (st + 4 < s1) gload(0, st + 4);
+      }
+      __syncthreads();
+    }
   }
-};
+  float *P = a.P + (size_t)blockIdx.x * OFF_WO;
+#pragma unroll
+  for (int i = 0; i < BN; ++i)
+#pragma unroll
+    for (int j = 0; j < BK; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const uint32_t n = wn * BN * 32 + 32 * i + 8 * (r >> 2) + 4 * lh + (r & 3);
+        const uint32_t k = wk * BK * 32 + 32 * j + l32;
+        P[off_w(l) + (size_t)n * K + k] = acc[i][j][r];
+      }
+  csum += __shfl_xor(csum, 1);
+  if ((tid & 1u) == 0) P[off_b(l) + (tid >> 1)] = csum;
+}
+
+__global__ __launch_bounds__(512) void k_mlp_wgrad(WGradArgs a) {
+  __shared__ __align__(16) uint16_t lds[WG_LDS];
+  const uint32_t l = blockIdx.y;
+  if (l == 0)
+    wgrad_body<K0>(a, 0, lds);
+  else
+    wgrad_body<HID>(a, l, lds);
+}
 
 // ---------------------------------------------------------------------------
 // small kernels
@@ -333,20 +520,31 @@ __device__ inline uint64_t mix64(uint64_t x) {
 // per map: 13 floats = three normalised site positions, four factors - 1
 constexpr int MAPP = 16;
 
-__device__ inline void write_features(uint16_t *x, const float *vxyz, const float *mp) {
-  for (int k = 0; k < 3; ++k) x[k] = f2bf(vxyz[k]);
-  for (int k = 0; k < 13; ++k) x[3 + k] = f2bf(mp[k]);
-  for (int k = NIN; k < K0; ++k) x[k] = 0;
+// the 16 inputs of one row into X0 (k-step layout; the padding columns
+// NIN..K0-1 are zero from the start and never written)
+__device__ inline void write_features(uint16_t *X0, size_t row, const float *vxyz, const float *mp) {
+  for (int k = 0; k < 3; ++k) X0[ks_off(row, k, K0)] = f2bf(vxyz[k]);
+  for (int k = 0; k < 13; ++k) X0[ks_off(row, 3 + k, K0)] = f2bf(mp[k]);
 }
 
-// training rows: random (map, vertex) samples
+__device__ inline void zero_features(uint16_t *X0, size_t row) {
+  for (int k = 0; k < NIN; ++k) X0[ks_off(row, k, K0)] = 0;
+}
+
+// training rows: random (map, vertex) samples; rows B..rows-1 (padding of
+// the last tile) are zero with target 0
 __global__ void k_sample(const float *coords, const float *mparams, const float *T, uint32_t nn,
-                         uint32_t nmaps, uint64_t seed, uint32_t B, float inv_ts, uint16_t *X0,
+                         uint32_t nmaps, uint64_t seed, uint32_t B, uint32_t rows, float inv_ts, uint16_t *X0,
                          float *tgt) {
-  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < B; m += gridDim.x * blockDim.x) {
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < rows; m += gridDim.x * blockDim.x) {
+    if (m >= B) {
+      zero_features(X0, m);
+      tgt[m] = 0.0f;
+      continue;
+    }
     const uint64_t h = mix64(seed ^ mix64((uint64_t)m + 1));
     const uint32_t j = (uint32_t)((h >> 32) % nmaps), v = (uint32_t)((h & 0xffffffffu) % nn);
-    write_features(X0 + (size_t)m * K0, coords + 3 * (size_t)v, mparams + MAPP * (size_t)j);
+    write_features(X0, m, coords + 3 * (size_t)v, mparams + MAPP * (size_t)j);
     tgt[m] = T[(size_t)j * nn + v] * inv_ts;
   }
 }
@@ -355,7 +553,7 @@ __global__ void k_sample(const float *coords, const float *mparams, const float 
 __global__ void k_map_rows(const float *coords, const float *mp, uint32_t nn, uint32_t rows,
                            uint16_t *X0) {
   for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < rows; m += gridDim.x * blockDim.x)
-    write_features(X0 + (size_t)m * K0, coords + 3 * (size_t)(m < nn ? m : 0), mp);
+    write_features(X0, m, coords + 3 * (size_t)(m < nn ? m : 0), mp);
 }
 
 // rows of several solves over one vertex sample: row r is solve r / m at
@@ -365,7 +563,7 @@ __global__ void k_sample_rows(const float *coords, const float *mp, const uint32
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
     const uint32_t q = r < nreal ? r : nreal - 1;
     const uint32_t j = q / m, v = sample[q - j * m];
-    write_features(X0 + (size_t)r * K0, coords + 3 * (size_t)v, mp + MAPP * (size_t)j);
+    write_features(X0, r, coords + 3 * (size_t)v, mp + MAPP * (size_t)j);
   }
 }
 
@@ -380,95 +578,10 @@ __global__ __launch_bounds__(256) void k_solve_max(const float *t, uint32_t m, u
   }
 }
 
-// given inputs (n x 16 floats) -> padded BF16 rows
+// given inputs (n x 16 floats) -> BF16 rows of X0 (rows n..rows-1 zero)
 __global__ void k_input_rows(const float *x, uint32_t n, uint32_t rows, uint16_t *X0) {
   for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < rows; m += gridDim.x * blockDim.x)
-    for (int k = 0; k < K0; ++k)
-      X0[(size_t)m * K0 + k] = (m < n && k < NIN) ? f2bf(x[(size_t)m * NIN + k]) : (uint16_t)0;
-}
-
-// output layer and squared error, one wave per row: y = H5 . w_out + b_out;
-// e = y - t; dZ5 = (2 e / B) w_out * SiLU'(Z5); gradients of w_out, b_out
-__global__ __launch_bounds__(256) void k_head_train(const uint16_t *H5, const uint16_t *Z5,
-                                                    const float *wo, const float *bo,
-                                                    const float *tgt, uint32_t B, float inv_b,
-                                                    uint16_t *dZ5, float *g_wo, float *g_bo,
-                                                    float *loss) {
-  __shared__ float s_gwo[HID];
-  __shared__ float s_misc[2];
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  s_gwo[tid] = 0.0f;
-  if (tid < 2) s_misc[tid] = 0.0f;
-  __syncthreads();
-  const uint32_t c = lane * 4;
-  const float w4[4] = {wo[c], wo[c + 1], wo[c + 2], wo[c + 3]};
-  float gw[4] = {0.0f, 0.0f, 0.0f, 0.0f}, gb = 0.0f, ls = 0.0f;
-  for (uint32_t row = blockIdx.x * 4 + wave; row < B; row += gridDim.x * 4) {
-    const u16x4 hv = *reinterpret_cast<const u16x4 *>(H5 + (size_t)row * HID + c);
-    const float h[4] = {bf2f(hv[0]), bf2f(hv[1]), bf2f(hv[2]), bf2f(hv[3])};
-    float p = h[0] * w4[0] + h[1] * w4[1] + h[2] * w4[2] + h[3] * w4[3];
-    for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off);
-    const float e = p + bo[0] - tgt[row];
-    const float dy = 2.0f * e * inv_b;
-    if (lane == 0) {
-      ls += e * e;
-      gb += dy;
-    }
-    const u16x4 zv = *reinterpret_cast<const u16x4 *>(Z5 + (size_t)row * HID + c);
-    u16x4 o;
-    for (int j = 0; j < 4; ++j) {
-      gw[j] += dy * h[j];
-      o[j] = f2bf(dy * w4[j] * dsilu(bf2f(zv[j])));
-    }
-    *reinterpret_cast<u16x4 *>(dZ5 + (size_t)row * HID + c) = o;
-  }
-  for (int j = 0; j < 4; ++j) atomicAdd(&s_gwo[c + j], gw[j]);
-  if (lane == 0) {
-    atomicAdd(&s_misc[0], gb);
-    atomicAdd(&s_misc[1], ls);
-  }
-  __syncthreads();
-  atomicAdd(&g_wo[tid], s_gwo[tid]);
-  if (tid == 0) {
-    atomicAdd(g_bo, s_misc[0]);
-    atomicAdd(loss, s_misc[1]);
-  }
-}
-
-// output layer for prediction: t = (H5 . w_out + b_out) * t_scale + delta
-__global__ __launch_bounds__(256) void k_head_infer(const uint16_t *H5, const float *wo, const float *bo,
-                                                    uint32_t n, float t_scale, float delta, float *out) {
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t c = lane * 4;
-  const float w4[4] = {wo[c], wo[c + 1], wo[c + 2], wo[c + 3]};
-  for (uint32_t row = blockIdx.x * 4 + wave; row < n; row += gridDim.x * 4) {
-    const u16x4 hv = *reinterpret_cast<const u16x4 *>(H5 + (size_t)row * HID + c);
-    float p = bf2f(hv[0]) * w4[0] + bf2f(hv[1]) * w4[1] + bf2f(hv[2]) * w4[2] + bf2f(hv[3]) * w4[3];
-    for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off);
-    if (lane == 0) out[row] = (p + bo[0]) * t_scale + delta;
-  }
-}
-
-// column sums of dZ (B x 256) into g_b (bias gradient)
-// column sums of dZ (B x 256) into g_b: a thread sums 8 columns of every
-// eighth row of the block's rows (16-byte loads), the block reduces in LDS
-__global__ __launch_bounds__(256) void k_colsum(const uint16_t *dZ, uint32_t B, float *g_b) {
-  __shared__ float part[8][HID];
-  const uint32_t tid = threadIdx.x, cg = tid & 31u, rl = tid >> 5;  // column group, row lane
-  float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll 4
-  for (uint32_t row = blockIdx.x * 8u + rl; row < B; row += gridDim.x * 8u) {
-    const u16x8 v = *reinterpret_cast<const u16x8 *>(dZ + (size_t)row * HID + cg * 8u);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) s[i] += bf2f(v[i]);
-  }
-#pragma unroll
-  for (int i = 0; i < 8; ++i) part[rl][cg * 8u + i] = s[i];
-  __syncthreads();
-  float t = 0.0f;
-#pragma unroll
-  for (int r = 0; r < 8; ++r) t += part[r][tid];
-  atomicAdd(&g_b[tid], t);
+    for (int k = 0; k < NIN; ++k) X0[ks_off(m, k, K0)] = m < n ? f2bf(x[(size_t)m * NIN + k]) : (uint16_t)0;
 }
 
 // sum of the batch-split partials
@@ -508,8 +621,9 @@ __global__ void k_adam(AdamArgs a) {
     for (int l = 0; l < NL; ++l)
       if (i >= off_w(l) && i < off_b(l)) {
         const size_t q = i - off_w(l);
-        a.wb[offb_w(l) + q] = f2bf(w);
-        if (l > 0) a.wbt[offbt(l) + (q % HID) * HID + q / HID] = f2bf(w);
+        const uint32_t K = l == 0 ? K0 : HID, n = (uint32_t)(q / K), k = (uint32_t)(q % K);
+        a.wb[offb_w(l) + frag_off(n, k, K)] = f2bf(w);
+        if (l > 0) a.wbt[offbt(l) + frag_off(k, n, HID)] = f2bf(w);
       }
   }
 }
@@ -519,8 +633,9 @@ __global__ void k_to_bf16(const float *w, uint16_t *wb, uint16_t *wbt) {
     for (int l = 0; l < NL; ++l)
       if (i >= off_w(l) && i < off_b(l)) {
         const size_t q = i - off_w(l);
-        wb[offb_w(l) + q] = f2bf(w[i]);
-        if (l > 0) wbt[offbt(l) + (q % HID) * HID + q / HID] = f2bf(w[i]);
+        const uint32_t K = l == 0 ? K0 : HID, n = (uint32_t)(q / K), k = (uint32_t)(q % K);
+        wb[offb_w(l) + frag_off(n, k, K)] = f2bf(w[i]);
+        if (l > 0) wbt[offbt(l) + frag_off(k, n, HID)] = f2bf(w[i]);
       }
 }
 
@@ -555,11 +670,11 @@ struct fim_nn {
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   uint16_t *wb = nullptr, *wbt = nullptr;
   int64_t t = 0;  // Adam steps taken
-  // activations (rows = cap)
+  // activations (rows = cap, a multiple of the chain kernel's tile), k-step layout
   uint32_t cap = 0;
-  uint16_t *X0 = nullptr, *Z[NL] = {}, *H[NL] = {}, *dZa = nullptr, *dZb = nullptr;
+  uint16_t *X0 = nullptr, *Z[NL - 1] = {}, *dZ[NL] = {};
   float *tgt = nullptr, *out = nullptr, *parts = nullptr, *loss = nullptr;
-  uint32_t nsplit = 1;
+  uint32_t nsplit = 1;  // row spans of the weight gradients (partial sums)
   // dataset
   float *coords = nullptr, *T = nullptr, *mparams = nullptr;
   int32_t nmaps = 0, max_maps = 0;
@@ -604,62 +719,58 @@ void ensure_rows(fim_nn *n, uint32_t rows) {
   throw Error(FIM_ELIMIT, "batch larger than the surrogate's row capacity");
 }
 
-// forward pass over `rows` rows of X0 (store_z keeps Z for backward)
-void forward(fim_nn *n, uint32_t rows, bool store_z) {
-  const bool big = rows % 128 == 0;
-  const dim3 g = big ? dim3(HID / 128, rows / 128, 1) : dim3(HID / 64, rows / 64, 1);
-  for (int l = 0; l < NL; ++l) {
-    const uint16_t *in = l == 0 ? n->X0 : n->H[l - 1];
-    const size_t K = l == 0 ? K0 : HID;
-    EpiFwd e{n->w + off_b(l), store_z ? n->Z[l] : nullptr, n->H[l]};
-    if (big)
-      k_gemm128<true, true, EpiFwd><<<g, 256, 0, n->stream>>>(in, K, 1, n->wb + offb_w(l), 1, K, (uint32_t)K, e);
-    else
-      k_gemm<true, true, EpiFwd><<<g, 256, 0, n->stream>>>(in, K, 1, n->wb + offb_w(l), 1, K, (uint32_t)K, e);
-  }
+// rows rounded up to whole tiles of the chain kernel
+uint32_t tile_rows(uint32_t rows) { return (rows + TR - 1) / TR * TR; }
+
+float t_scale(const fim_nn *n) { return (float)(n->o.t_scale > 0 ? n->o.t_scale : 1e5); }
+
+// predictions for rows 0..nreal-1 of X0 (filled up to tile_rows(nreal)):
+// out[r] = y_r t_scale + delta
+void infer(fim_nn *n, uint32_t nreal, float delta, float *out) {
+  ChainArgs a{};
+  a.X0 = n->X0;
+  a.wb = n->wb;
+  a.wbt = n->wbt;
+  a.w = n->w;
+  a.out = out;
+  a.nreal = nreal;
+  a.t_scale = t_scale(n);
+  a.delta = delta;
+  k_mlp_chain<false><<<tile_rows(nreal) / TR, NT, CHAIN_LDS, n->stream>>>(a);
   HIPCHECK(hipGetLastError());
 }
 
-// backward pass for `rows` rows (targets in n->tgt); gradients in n->g
-void backward(fim_nn *n, uint32_t rows) {
-  HIPCHECK(hipMemsetAsync(n->g, 0, NPAD * sizeof(float), n->stream));
+// forward and backward pass over rows 0..B-1 of X0 (targets in n->tgt; rows
+// up to tile_rows(B) filled, as padding): gradient in n->g, loss sum in
+// n->loss
+void train_pass(fim_nn *n, uint32_t B) {
+  const uint32_t rows = tile_rows(B);
+  HIPCHECK(hipMemsetAsync(n->g + OFF_WO, 0, (NPAD - OFF_WO) * sizeof(float), n->stream));
   HIPCHECK(hipMemsetAsync(n->loss, 0, sizeof(float), n->stream));
-  k_head_train<<<std::min<uint32_t>(rows / 4, 1024), 256, 0, n->stream>>>(
-      n->H[NL - 1], n->Z[NL - 1], n->w + OFF_WO, n->w + OFF_BO, n->tgt, rows, 1.0f / rows, n->dZa,
-      n->g + OFF_WO, n->g + OFF_BO, n->loss);
-  uint16_t *dz = n->dZa, *dzp = n->dZb;
-  // batch splits of the weight gradient: spans of >= 32 rows
-  uint32_t S = std::min<uint32_t>(n->nsplit, rows / 32);
-  while (rows % (S * 32)) --S;
-  const uint32_t span = rows / S;
-  for (int l = NL - 1; l >= 0; --l) {
-    const uint16_t *hin = l == 0 ? n->X0 : n->H[l - 1];
-    const uint32_t K = l == 0 ? K0 : HID;
-    // dW_l[n][k] = sum_m dZ[m][n] hin[m][k]: A(r=n, k=m) = dZ (contiguous in r),
-    // B(k=m, c=k) = hin (contiguous in c)
-    EpiPart ep{n->parts, K};
-    if (K % 128 == 0)
-      k_gemm128<false, false, EpiPart><<<dim3(K / 128, HID / 128, S), 256, 0, n->stream>>>(dz, 1, HID, hin, K, 1,
-                                                                                           span, ep);
-    else
-      k_gemm<false, false, EpiPart><<<dim3(K / 64, HID / 64, S), 256, 0, n->stream>>>(dz, 1, HID, hin, K, 1, span,
-                                                                                       ep);
-    k_reduce_parts<<<grid_for((size_t)HID * K), 256, 0, n->stream>>>(n->parts, S, (size_t)HID * K,
-                                                                     n->g + off_w(l));
-    k_colsum<<<std::min<uint32_t>(rows / 8, 512), 256, 0, n->stream>>>(dz, rows, n->g + off_b(l));
-    if (l > 0) {
-      // dZ_{l-1}[m][k] = sum_n dZ[m][n] W_l[n][k] * SiLU'(Z_{l-1})
-      EpiBwd eb{n->Z[l - 1], dzp};
-      // B(k = n, c = k') = W_l[n][k'] = WT_l[k'][n]: contiguous in k
-      if (rows % 128 == 0)
-        k_gemm128<true, true, EpiBwd><<<dim3(HID / 128, rows / 128, 1), 256, 0, n->stream>>>(
-            dz, HID, 1, n->wbt + offbt(l), 1, HID, HID, eb);
-      else
-        k_gemm<true, true, EpiBwd><<<dim3(HID / 64, rows / 64, 1), 256, 0, n->stream>>>(
-            dz, HID, 1, n->wbt + offbt(l), 1, HID, HID, eb);
-      std::swap(dz, dzp);
-    }
+  ChainArgs a{};
+  a.X0 = n->X0;
+  a.wb = n->wb;
+  a.wbt = n->wbt;
+  a.w = n->w;
+  a.tgt = n->tgt;
+  for (int l = 0; l < NL - 1; ++l) a.Z[l] = n->Z[l];
+  for (int l = 0; l < NL; ++l) a.dZ[l] = n->dZ[l];
+  a.g = n->g;
+  a.loss = n->loss;
+  a.nreal = B;
+  a.inv_b = 1.0f / (float)B;
+  k_mlp_chain<true><<<rows / TR, NT, CHAIN_LDS, n->stream>>>(a);
+  // weight gradients: S spans of whole 32-row steps, one partial sum each
+  const uint32_t S = std::min<uint32_t>(n->nsplit, rows / 32);
+  WGradArgs wa;
+  for (int l = 0; l < NL; ++l) {
+    wa.dZ[l] = n->dZ[l];
+    wa.Hin[l] = l == 0 ? n->X0 : n->Z[l - 1];
   }
+  wa.P = n->parts;
+  wa.steps = rows / 32;
+  k_mlp_wgrad<<<dim3(S, NL), 512, 0, n->stream>>>(wa);
+  k_reduce_parts<<<grid_for(OFF_WO), 256, 0, n->stream>>>(n->parts, S, OFF_WO, n->g);
   HIPCHECK(hipGetLastError());
 }
 
@@ -681,8 +792,6 @@ void adam(fim_nn *n) {
   k_adam<<<grid_for(NPAD), 256, 0, n->stream>>>(a);
   HIPCHECK(hipGetLastError());
 }
-
-float t_scale(const fim_nn *n) { return (float)(n->o.t_scale > 0 ? n->o.t_scale : 1e5); }
 
 // logical <-> device parameter order
 void to_device_order(const float *logical, std::vector<float> &dev) {
@@ -731,17 +840,13 @@ int guarded(F &&f) {
 
 // prediction for all vertices of one solve into dst (device, nn floats)
 void predict_map_device(fim_nn *n, const std::vector<float> &mp, float delta, float *dst) {
-  const uint32_t rows = (n->nn + 127u) & ~127u;  // 128-row tiles
+  const uint32_t rows = tile_rows(n->nn);
   ensure_rows(n, rows);
   HIPCHECK(hipMemcpyAsync(n->mparams + (size_t)n->max_maps * MAPP, mp.data(), MAPP * sizeof(float),
                           hipMemcpyHostToDevice, n->stream));
   k_map_rows<<<grid_for(rows), 256, 0, n->stream>>>(n->coords, n->mparams + (size_t)n->max_maps * MAPP,
                                                     n->nn, rows, n->X0);
-  forward(n, rows, false);
-  k_head_infer<<<std::min<uint32_t>(rows / 4, 4096), 256, 0, n->stream>>>(n->H[NL - 1], n->w + OFF_WO,
-                                                                          n->w + OFF_BO, n->nn,
-                                                                          t_scale(n), delta, dst);
-  HIPCHECK(hipGetLastError());
+  infer(n, n->nn, delta, dst);
 }
 
 }  // namespace
@@ -781,8 +886,10 @@ int fim_nn_create(const fim_mesh *mesh, const fim_nn_opts *o, fim_nn **out) {
       n->max_maps = o->max_maps > 0 ? o->max_maps : 4096;
       const uint32_t mb = o->max_batch > 0 ? (uint32_t)o->max_batch : 65536u;
       if (mb % 64) throw Error(FIM_EINVAL, "max_batch must be a multiple of 64");
-      n->cap = std::max<uint32_t>(mb, (n->nn + 127u) & ~127u);
-      n->nsplit = 128;  // batch splits of the weight gradients: 4 x 128 tiles of 128 x 128
+      n->cap = std::max<uint32_t>(tile_rows(mb), tile_rows(n->nn));
+      // row spans of the weight gradients: 6 x 40 workgroups fill the CUs
+      // once (32 / 48 / 64 spans measured 4-8 % slower per training step)
+      n->nsplit = 40;
       // parameters: He-uniform hidden layers, small output layer, zero biases
       std::vector<float> w(NPAD, 0.0f);
       uint64_t st = o->seed ^ 0x5DEECE66Dull;
@@ -814,15 +921,16 @@ int fim_nn_create(const fim_mesh *mesh, const fim_nn_opts *o, fim_nn **out) {
       k_to_bf16<<<grid_for(NPAD), 256, 0, n->stream>>>(n->w, n->wb, n->wbt);
       HIPCHECK(hipGetLastError());
       n->X0 = nn_alloc<uint16_t>(n, (size_t)n->cap * K0);
-      for (int l = 0; l < NL; ++l) {
-        n->Z[l] = nn_alloc<uint16_t>(n, (size_t)n->cap * HID);
-        n->H[l] = nn_alloc<uint16_t>(n, (size_t)n->cap * HID);
-      }
-      n->dZa = nn_alloc<uint16_t>(n, (size_t)n->cap * HID);
-      n->dZb = nn_alloc<uint16_t>(n, (size_t)n->cap * HID);
+      HIPCHECK(hipMemset(n->X0, 0, (size_t)n->cap * K0 * sizeof(uint16_t)));  // padding columns stay 0
+      for (int l = 0; l < NL - 1; ++l) n->Z[l] = nn_alloc<uint16_t>(n, (size_t)n->cap * HID);
+      for (int l = 0; l < NL; ++l) n->dZ[l] = nn_alloc<uint16_t>(n, (size_t)n->cap * HID);
       n->tgt = nn_alloc<float>(n, n->cap);
       n->out = nn_alloc<float>(n, n->cap);
-      n->parts = nn_alloc<float>(n, (size_t)n->nsplit * HID * HID);
+      n->parts = nn_alloc<float>(n, (size_t)n->nsplit * OFF_WO);
+      HIPCHECK(hipFuncSetAttribute((const void *)k_mlp_chain<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)CHAIN_LDS));
+      HIPCHECK(hipFuncSetAttribute((const void *)k_mlp_chain<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)CHAIN_LDS));
       n->loss = nn_alloc<float>(n, 1);
       n->coords = nn_alloc<float>(n, 3 * (size_t)n->nn);
       HIPCHECK(hipMemcpy(n->coords, n->coords_h.data(), 3 * (size_t)n->nn * sizeof(float), hipMemcpyHostToDevice));
@@ -838,6 +946,7 @@ int fim_nn_create(const fim_mesh *mesh, const fim_nn_opts *o, fim_nn **out) {
 }
 
 void fim_nn_destroy(fim_nn *n) { nn_destroy(n); }
+
 
 int fim_nn_get_params(fim_nn *n, float *params) {
   return guarded([&] {
@@ -942,7 +1051,7 @@ int fim_nn_gradient(fim_nn *n, const float *x, const float *t, int32_t b, double
   return guarded([&] {
     if (!n || !x || !t || !loss || !grad) throw Error(FIM_EINVAL, "null argument");
     if (b <= 0 || b % 64) throw Error(FIM_EINVAL, "batch must be a positive multiple of 64");
-    ensure_rows(n, (uint32_t)b);
+    ensure_rows(n, tile_rows((uint32_t)b));
     HIPCHECK(hipSetDevice(n->dev));
     std::vector<float> ts(b);
     for (int32_t i = 0; i < b; ++i) ts[i] = t[i] / t_scale(n);
@@ -951,9 +1060,9 @@ int fim_nn_gradient(fim_nn *n, const float *x, const float *t, int32_t b, double
     try {
       HIPCHECK(hipMemcpyAsync(xd, x, (size_t)b * NIN * sizeof(float), hipMemcpyHostToDevice, n->stream));
       HIPCHECK(hipMemcpyAsync(n->tgt, ts.data(), (size_t)b * sizeof(float), hipMemcpyHostToDevice, n->stream));
-      k_input_rows<<<grid_for((size_t)b), 256, 0, n->stream>>>(xd, (uint32_t)b, (uint32_t)b, n->X0);
-      forward(n, (uint32_t)b, true);
-      backward(n, (uint32_t)b);
+      k_input_rows<<<grid_for(tile_rows((uint32_t)b)), 256, 0, n->stream>>>(xd, (uint32_t)b,
+                                                                            tile_rows((uint32_t)b), n->X0);
+      train_pass(n, (uint32_t)b);
       std::vector<float> dev(NPAD);
       float l = 0.0f;
       HIPCHECK(hipMemcpyAsync(dev.data(), n->g, NPAD * sizeof(float), hipMemcpyDeviceToHost, n->stream));
@@ -974,15 +1083,15 @@ int fim_nn_train(fim_nn *n, int32_t steps, int32_t batch, uint64_t seed, double 
     if (!n) throw Error(FIM_EINVAL, "null argument");
     if (steps < 0 || batch <= 0 || batch % 64) throw Error(FIM_EINVAL, "bad training arguments");
     if (n->nmaps == 0) throw Error(FIM_ESTATE, "surrogate dataset is empty");
-    ensure_rows(n, (uint32_t)batch);
+    const uint32_t rows = tile_rows((uint32_t)batch);
+    ensure_rows(n, rows);
     HIPCHECK(hipSetDevice(n->dev));
     float l = 0.0f;
     for (int32_t s = 0; s < steps; ++s) {
-      k_sample<<<grid_for((size_t)batch), 256, 0, n->stream>>>(
-          n->coords, n->mparams, n->T, n->nn, (uint32_t)n->nmaps, seed * 0x100000001B3ull + (uint64_t)s,
-          (uint32_t)batch, 1.0f / t_scale(n), n->X0, n->tgt);
-      forward(n, (uint32_t)batch, true);
-      backward(n, (uint32_t)batch);
+      k_sample<<<grid_for(rows), 256, 0, n->stream>>>(n->coords, n->mparams, n->T, n->nn, (uint32_t)n->nmaps,
+                                                      seed * 0x100000001B3ull + (uint64_t)s, (uint32_t)batch,
+                                                      rows, 1.0f / t_scale(n), n->X0, n->tgt);
+      train_pass(n, (uint32_t)batch);
       adam(n);
     }
     HIPCHECK(hipMemcpyAsync(&l, n->loss, sizeof(float), hipMemcpyDeviceToHost, n->stream));
@@ -995,8 +1104,8 @@ int fim_nn_predict(fim_nn *n, const float *x, int64_t count, float *t) {
   return guarded([&] {
     if (!n || (!x && count > 0) || (!t && count > 0) || count < 0) throw Error(FIM_EINVAL, "null argument");
     if (count == 0) return;
-    const uint32_t r128 = (uint32_t)((count + 127) & ~int64_t(127));
-    const uint32_t rows = r128 <= n->cap ? r128 : (uint32_t)((count + 63) & ~int64_t(63));
+    if (count > (int64_t)n->cap) throw Error(FIM_ELIMIT, "batch larger than the surrogate's row capacity");
+    const uint32_t rows = tile_rows((uint32_t)count);
     ensure_rows(n, rows);
     HIPCHECK(hipSetDevice(n->dev));
     float *xd = nullptr;
@@ -1004,10 +1113,7 @@ int fim_nn_predict(fim_nn *n, const float *x, int64_t count, float *t) {
     try {
       HIPCHECK(hipMemcpyAsync(xd, x, (size_t)count * NIN * sizeof(float), hipMemcpyHostToDevice, n->stream));
       k_input_rows<<<grid_for(rows), 256, 0, n->stream>>>(xd, (uint32_t)count, rows, n->X0);
-      forward(n, rows, false);
-      k_head_infer<<<std::min<uint32_t>(rows / 4, 4096), 256, 0, n->stream>>>(
-          n->H[NL - 1], n->w + OFF_WO, n->w + OFF_BO, (uint32_t)count, t_scale(n), 0.0f, n->out);
-      HIPCHECK(hipGetLastError());
+      infer(n, (uint32_t)count, 0.0f, n->out);
       HIPCHECK(hipMemcpyAsync(t, n->out, (size_t)count * sizeof(float), hipMemcpyDeviceToHost, n->stream));
       HIPCHECK(hipStreamSynchronize(n->stream));
     } catch (...) {
@@ -1051,13 +1157,10 @@ int fim_nn_predict_max(fim_nn *n, int32_t count, const int64_t *sites, const flo
       for (uint32_t j0 = 0; j0 < (uint32_t)count; j0 += per) {
         const uint32_t nj = std::min<uint32_t>(per, (uint32_t)count - j0);
         const uint32_t nreal = nj * (uint32_t)m;
-        uint32_t rows = (nreal + 127u) & ~127u;
-        if (rows > n->cap) rows = (nreal + 63u) & ~63u;
+        const uint32_t rows = tile_rows(nreal);  // <= cap: cap is a whole number of tiles
         k_sample_rows<<<grid_for(rows), 256, 0, n->stream>>>(n->coords, md + (size_t)j0 * MAPP, sd, (uint32_t)m,
                                                              nreal, rows, n->X0);
-        forward(n, rows, false);
-        k_head_infer<<<std::min<uint32_t>(rows / 4, 4096), 256, 0, n->stream>>>(
-            n->H[NL - 1], n->w + OFF_WO, n->w + OFF_BO, nreal, t_scale(n), 0.0f, n->out);
+        infer(n, nreal, 0.0f, n->out);
         k_solve_max<<<std::min<uint32_t>((nj + 3) / 4, 4096), 256, 0, n->stream>>>(n->out, (uint32_t)m, nj,
                                                                                   od + j0);
         HIPCHECK(hipGetLastError());

@@ -43,7 +43,7 @@ def _chunks(seq, n):
         yield i, seq[i:i + n]
 
 
-def window_sample(mesh, k=2040, seed=0):
+def window_sample(mesh, k=1000, seed=0):
     """Vertices at which the latest activation of a solve is estimated: the
     vertex nearest each corner of the mesh's bounding box and k seeded
     random others (the latest activation lies on the boundary, mostly at a

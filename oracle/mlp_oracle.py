@@ -7,9 +7,11 @@ tests/ may import it.  The network has no counterpart in the reference.
 
 With ``bf16=True`` every value the GPU rounds to BF16 is rounded here at the
 same place (inputs, hidden-layer weights, pre-activations Z kept for the
-backward pass, activations H, backward dZ), so the two differ only by
-accumulation order (FP32 on the GPU).  With ``bf16=False`` the pass is
-exact FP64, and its gradient is pinned by central finite differences.
+backward pass, activations H, backward dZ; the weight gradients take
+H = SiLU(Z) of the stored Z, rounded), so the two differ only by
+accumulation order (FP32 on the GPU) and the hardware exponential and
+reciprocal.  With ``bf16=False`` the pass is exact FP64, and its gradient is
+pinned by central finite differences.
 
 Adam (``adam_step``) is Kingma & Ba's Algorithm 1 with bias-corrected
 moments; it is pinned by closed forms (first step, constant and alternating
@@ -88,7 +90,8 @@ def loss_and_grad(p, x, t, bf16_emulation=True):
     dz = rnd(dy[:, None] * wo[None, :] * dsilu(Z5))
     grads = [None] * NL
     for layer in range(NL - 1, -1, -1):
-        hin = cache[layer] if layer == 0 else cache[layer][1]
+        # the weight gradient recomputes H_{l-1} from the stored Z_{l-1}
+        hin = cache[layer] if layer == 0 else rnd(silu(cache[layer][0]))
         gW = dz.T @ hin
         gb = dz.sum(axis=0)
         grads[layer] = (gW, gb)
