@@ -479,6 +479,7 @@ def main():
             "local_iters": args.local_iters,
             "window_us": args.window,
             "quantile": args.quantile,
+            "sample": args.sample,
             "sample_vertices": len(sample),
         },
         "max_abs_dT_us": max_dt,

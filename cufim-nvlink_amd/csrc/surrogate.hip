@@ -388,52 +388,68 @@ struct WGradArgs {
                             // [s steps / S, (s + 1) steps / S), S = gridDim.x
 };
 
-constexpr int WG_LDS = 2 * 2 * 2 * HID * 16;  // BF16 values: 2 buffers x (dZ + H) x 32 rows x 256
+// LDS of the weight-gradient kernel: two buffers of dZ and H, each two
+// 16-row k-steps x 256 columns; a column's 16 rows take 24 values (48 B), so
+// the 32 lanes of a fragment read hit distinct banks
+constexpr int KSP = 24;
+constexpr int WG_LDS = 2 * 2 * 2 * HID * KSP;  // BF16 values
 
 template <int K>
 __device__ inline void wgrad_body(const WGradArgs &a, uint32_t l, uint16_t *lds) {
   constexpr int WN = K == HID ? 4 : 8, WK = 8 / WN;
   constexpr int BN = HID / WN / 32, BK = K / WK / 32;
-  constexpr uint32_t BUF = 2 * 2 * HID * 16;  // one buffer: dZ (2 x HID x 16) then H (2 x K x 16)
+  constexpr uint32_t BUF = 2 * 2 * HID * KSP;  // one buffer: dZ (2 x HID x KSP) then H (2 x K x KSP)
   constexpr uint32_t HPC = 2 * K * 16 / 8;    // 16-byte pieces of H per step
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t wn = wave / WK, wk = wave % WK, l32 = lane & 31u, lh = lane >> 5;
   const uint32_t s0 = (uint32_t)((uint64_t)blockIdx.x * a.steps / gridDim.x);
   const uint32_t s1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * a.steps / gridDim.x);
+  const uint32_t n = s1 - s0;                   // >= 1: the host launches at most a.steps spans
+  const uint32_t npad = (n + 5) / 6 * 6;        // steps run: the last ones stage zeros
   const uint16_t *dz = a.dZ[l], *hz = a.Hin[l];  // k-step blocks of consecutive rows are contiguous
-  // two register slots of staged global loads: step st + 2 is loaded while
-  // step st is multiplied and step st + 1 is stored to LDS
-  u16x8 rz[2][2], rh[2][2];
+  // Three register slots of global loads: relative step i + 4 is loaded while
+  // step i is multiplied and step i + 1 is staged.  Every load is issued (the
+  // step index clamped), so the number of loads in flight is the same on
+  // every path and each wait leaves the two younger steps in flight.
+  u16x8 rz[3][2], rh[3][2];
   float csum = 0.0f;  // column sum of dZ for column tid / 2
-  auto gload = [&](int slot, uint32_t st) {
+  auto gload = [&](int slot, uint32_t i) {
+    const uint32_t st = s0 + (i < n ? i : n - 1);
     const u16x8 *sz = reinterpret_cast<const u16x8 *>(dz + (size_t)st * 32 * HID);
     const u16x8 *sh = reinterpret_cast<const u16x8 *>(hz + (size_t)st * 32 * K);
     rz[slot][0] = sz[tid];
     rz[slot][1] = sz[tid + 512];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const uint32_t p = tid + 512u * u;
-      if (p < HPC) rh[slot][u] = sh[p];
-    }
+    for (int u = 0; u < 2; ++u) rh[slot][u] = sh[(tid + 512u * u) % HPC];
   };
-  auto lstore = [&](int slot, uint32_t buf) {
-    u16x8 *bz = reinterpret_cast<u16x8 *>(lds + buf * BUF);
-    u16x8 *bh = reinterpret_cast<u16x8 *>(lds + buf * BUF + 2 * HID * 16);
-    bz[tid] = rz[slot][0];
-    bz[tid + 512] = rz[slot][1];
+  // stages relative step i (zeros beyond the span) into LDS buffer buf
+  auto lstore = [&](int slot, uint32_t buf, uint32_t i) {
+    const uint32_t msk = i < n ? 0xffffffffu : 0u;  // zeros beyond the span (SiLU(0) = 0)
+    auto masked = [&](u16x8 v) {
+      uint4 w = __builtin_bit_cast(uint4, v);
+      w.x &= msk;
+      w.y &= msk;
+      w.z &= msk;
+      w.w &= msk;
+      return __builtin_bit_cast(u16x8, w);
+    };
+    const u16x8 z0 = masked(rz[slot][0]), z1 = masked(rz[slot][1]);
+    // piece p of a step (16 B: 8 rows of one column) goes to column p / 2 of
+    // its k-step, half p % 2
+    uint16_t *bz = lds + buf * BUF, *bh = lds + buf * BUF + 2 * HID * KSP;
+    *reinterpret_cast<u16x8 *>(bz + (tid >> 1) * KSP + 8 * (tid & 1u)) = z0;
+    *reinterpret_cast<u16x8 *>(bz + (HID + (tid >> 1)) * KSP + 8 * (tid & 1u)) = z1;
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) csum += bf2f(rz[slot][u][j]);
+    for (int j = 0; j < 8; ++j) csum += bf2f(z0[j]) + bf2f(z1[j]);
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const uint32_t p = tid + 512u * u;
       if (p < HPC) {
-        u16x8 v = rh[slot][u];
+        u16x8 v = masked(rh[slot][u]);
         if (K == HID)
 #pragma unroll
           for (int j = 0; j < 8; ++j) v[j] = f2bf(silu(bf2f(v[j])));
-        bh[p] = v;
+        *reinterpret_cast<u16x8 *>(bh + (p >> 1) * KSP + 8 * (p & 1u)) = v;
       }
     }
   };
@@ -445,41 +461,35 @@ __device__ inline void wgrad_body(const WGradArgs &a, uint32_t l, uint16_t *lds)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
   auto product = [&](uint32_t buf) {
-    const uint16_t *bz = lds + buf * BUF, *bh = lds + buf * BUF + 2 * HID * 16;
+    const uint16_t *bz = lds + buf * BUF, *bh = lds + buf * BUF + 2 * HID * KSP;
 #pragma unroll
     for (int u = 0; u < 2; ++u) {  // two 16-row k-steps
       u16x8 fa[BN], fb[BK];
 #pragma unroll
       for (int i = 0; i < BN; ++i)
-        fa[i] = *reinterpret_cast<const u16x8 *>(bz + ((u * HID + wn * BN * 32 + 32 * i + l32) << 4) + 8 * lh);
+        fa[i] = *reinterpret_cast<const u16x8 *>(bz + (u * HID + wn * BN * 32 + 32 * i + l32) * KSP + 8 * lh);
 #pragma unroll
       for (int j = 0; j < BK; ++j)
-        fb[j] = *reinterpret_cast<const u16x8 *>(bh + ((u * K + wk * BK * 32 + 32 * j + l32) << 4) + 8 * lh);
+        fb[j] = *reinterpret_cast<const u16x8 *>(bh + (u * K + wk * BK * 32 + 32 * j + l32) * KSP + 8 * lh);
 #pragma unroll
       for (int i = 0; i < BN; ++i)
 #pragma unroll
         for (int j = 0; j < BK; ++j) acc[i][j] = mfma(fa[i], fb[j], acc[i][j]);
     }
   };
-  if (s0 < s1) {
-    gload(0, s0);
-    lstore(0, 0);
-    if (s0 + 1 < s1) gload(1, s0 + 1);
-    if (s0 + 2 < s1) gload(0, s0 + 2);
-    __syncthreads();
-    for (uint32_t st = s0; st < s1; st += 2) {
-      product(0);
-      if (st + 1 < s1) {
-        lstore(1, 1);
-        if (st + 3 < s1) gload(1, st + 3);
-      }
-      __syncthreads();
-      if (st + 1 >= s1) break;
-      product(1);
-      if (st + 2 < s1) {
-        lstore(0, 0);
-        if (st + 4 < s1) gload(0, st + 4);
-      }
+  // relative steps 6 m + r use LDS buffer r % 2 and register slot r % 3
+  gload(0, 0);
+  lstore(0, 0, 0);
+  gload(1, 1);
+  gload(2, 2);
+  gload(0, 3);
+  __syncthreads();
+  for (uint32_t i0 = 0; i0 < npad; i0 += 6) {
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      product((uint32_t)(r & 1));
+      lstore((r + 1) % 3, (uint32_t)((r + 1) & 1), i0 + r + 1);
+      gload((r + 1) % 3, i0 + r + 4);
       __syncthreads();
     }
   }
@@ -499,7 +509,8 @@ __device__ inline void wgrad_body(const WGradArgs &a, uint32_t l, uint16_t *lds)
 }
 
 __global__ __launch_bounds__(512) void k_mlp_wgrad(WGradArgs a) {
-  __shared__ __align__(16) uint16_t lds[WG_LDS];
+  extern __shared__ __align__(16) unsigned char wg_lds[];
+  uint16_t *lds = reinterpret_cast<uint16_t *>(wg_lds);
   const uint32_t l = blockIdx.y;
   if (l == 0)
     wgrad_body<K0>(a, 0, lds);
@@ -769,7 +780,7 @@ void train_pass(fim_nn *n, uint32_t B) {
   }
   wa.P = n->parts;
   wa.steps = rows / 32;
-  k_mlp_wgrad<<<dim3(S, NL), 512, 0, n->stream>>>(wa);
+  k_mlp_wgrad<<<dim3(S, NL), 512, WG_LDS * 2, n->stream>>>(wa);
   k_reduce_parts<<<grid_for(OFF_WO), 256, 0, n->stream>>>(n->parts, S, OFF_WO, n->g);
   HIPCHECK(hipGetLastError());
 }
@@ -931,6 +942,8 @@ int fim_nn_create(const fim_mesh *mesh, const fim_nn_opts *o, fim_nn **out) {
                                    (int)CHAIN_LDS));
       HIPCHECK(hipFuncSetAttribute((const void *)k_mlp_chain<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)CHAIN_LDS));
+      HIPCHECK(hipFuncSetAttribute((const void *)k_mlp_wgrad, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   WG_LDS * 2));
       n->loss = nn_alloc<float>(n, 1);
       n->coords = nn_alloc<float>(n, 3 * (size_t)n->nn);
       HIPCHECK(hipMemcpy(n->coords, n->coords_h.data(), 3 * (size_t)n->nn * sizeof(float), hipMemcpyHostToDevice));
