@@ -55,14 +55,17 @@ def test_forward_matches_oracle(small_slab):
     nn.close()
 
 
-def test_gradient_matches_oracle(small_slab):
+@pytest.mark.parametrize("b", [256, 2112])
+def test_gradient_matches_oracle(small_slab, b):
+    """b = 256: four row tiles, eight weight-gradient spans of one 32-row
+    step; b = 2112: 33 tiles, 66 steps over 40 spans of one or two steps."""
     _, m = small_slab
     nn = F.Surrogate(m, seed=1)
     p = M.init_params(7)
     nn.set_params(p)
     rng = np.random.default_rng(1)
-    x = rng.uniform(-1, 1, (256, M.NIN)).astype(np.float32)
-    t = rng.uniform(0.0, 2.0 * TS, 256).astype(np.float32)
+    x = rng.uniform(-1, 1, (b, M.NIN)).astype(np.float32)
+    t = rng.uniform(0.0, 2.0 * TS, b).astype(np.float32)
     loss, g = nn.gradient(x, t)
     rl, rg = M.loss_and_grad(p, x, t.astype(np.float64) / TS)
     assert abs(loss - rl) <= 1e-2 * rl
