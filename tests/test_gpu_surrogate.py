@@ -199,13 +199,16 @@ def test_training_and_warm_start_reach_the_cold_fixed_point(small_slab):
     nn.close()
 
 
-def test_predicted_maximum_equals_the_map_maximum(small_slab):
+@pytest.mark.parametrize("max_batch", [0, 64])
+def test_predicted_maximum_equals_the_map_maximum(small_slab, max_batch):
     """fim_nn_predict_max over a vertex sample equals the largest value of
     fim_nn_predict_map at those vertices (the same rows through the same
-    kernels, so bit for bit), for several solves in one call"""
+    kernels, so bit for bit), for several solves in one call; with the
+    smallest row capacity (the mesh's vertices) the solves take several
+    forward passes"""
     from fimpy import warmstart as WS
     cfg, m = small_slab
-    nn = F.Surrogate(m, seed=3)
+    nn = F.Surrogate(m, seed=3, max_batch=max_batch)
     nn.set_params(M.init_params(11))
     solves = _draw(cfg, 37, 5)
     sample = WS.window_sample(m, k=500, seed=1)
