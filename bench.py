@@ -263,11 +263,16 @@ def main():
                        local_iters=args.local_iters, window=args.window)
         st = WS.windowed_batch(acc, test, np.load(args.windows_in))
         acc.close()
+        # and the reference algorithm's windows (compat mode), for its counters
+        compat = None
+        if args.compat_steps > 0:
+            compat, _ = compat_leg(args, W.box(cfg, args.seed), W.pacing_sites(cfg, 5, args.seed * 100003))
         print(json.dumps({"accounting_only": True, "roofline": {
             "launches": st["compute_launches"], "launches_skipped": 0,
             "avg_launch_us": 1e3 * st["compute_ms"] / max(st["compute_launches"], 1),
             "alg_bytes_per_launch": st["alg_bytes"] / max(st["compute_launches"], 1),
-            "vertex_updates_per_launch": st["vertex_updates"] / max(st["compute_launches"], 1)}}), flush=True)
+            "vertex_updates_per_launch": st["vertex_updates"] / max(st["compute_launches"], 1)},
+            "compat": compat}), flush=True)
         return
 
     # ---- surrogate: training maps and training (set-up) ----------------------
