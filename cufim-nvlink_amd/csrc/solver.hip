@@ -13,8 +13,9 @@
 //    from L2, and candidates merge with LDS atomicMin on the float bit
 //    pattern (times are >= 0, so the IEEE order equals the unsigned order).
 //    A work item writes its lowered interface copies as (entry, time)
-//    records into its own slot range (no global atomics); a second kernel
-//    (k_conv_sync), one wave per work unit, pushes them to the other copies
+//    records into its own slot range and a descriptor (item, record count)
+//    into its work-unit slot (no global atomics); a second kernel
+//    (k_conv_sync), one wave per item slot, pushes them to the other copies
 //    of the vertex with global atomicMin, marks those dirty and queues their
 //    submesh on the next list; a third (k_conv_plan) cuts the next list into
 //    work units.  The kernels read only their own list, so results are
@@ -84,6 +85,8 @@ struct WorkList {
   uint32_t *bins;   // nsub x ninst instance ids
   uint32_t *plan;   // work units
   uint32_t *cnt;    // [0] units, [1] queued pairs, [2..9] per-XCD unit counters
+  uint2 *desc;      // push descriptors, slot u * unit + q for item q of unit u:
+                    // (instance * nsub + submesh, push records; 0 = none)
 };
 
 constexpr uint32_t kXcd = 8;
@@ -104,6 +107,13 @@ __device__ inline void requeue(const WorkList &l, uint32_t *act, uint32_t inst, 
   act[(size_t)inst * nsub + s] = 1u;
   const uint32_t j = atomicAdd(&l.bcnt[s], 1u);
   l.bins[(size_t)s * ninst + j] = inst;
+}
+
+// Every item of a list leaves its push descriptor in its unit slot
+// (instance * nsub + submesh, push records of its activation, 0 if none), so
+// the interface push finds an item's records with one load.
+__device__ inline void push_descriptor(const WorkList &l, uint32_t slot, size_t entry, uint32_t n) {
+  l.desc[slot] = make_uint2((uint32_t)entry, n);
 }
 
 struct UnitRange {
@@ -127,13 +137,15 @@ struct UnitCursor {
   uint32_t U, G, x, y, d, ninst, unit, maxch;
   UnitRange R;
   uint32_t s, base, n_in, i;
+  uint32_t uidx = 0;  // unit of the current item
+  uint2 *desc;
   bool first;
 
   __device__ UnitCursor(const WorkList &l, uint32_t *lds_slot, uint32_t ninst_, uint32_t unit_,
                         uint32_t maxch_)
       : bcnt(l.bcnt), bins(l.bins), plan(l.plan), grab(l.cnt + 2), slot(lds_slot), U(l.cnt[0]),
         G(gridDim.x), x(blockIdx.x % kXcd), y(blockIdx.x % kXcd), d(0), ninst(ninst_),
-        unit(unit_), maxch(maxch_), s(0), base(0), n_in(0), i(0), first(true) {
+        unit(unit_), maxch(maxch_), s(0), base(0), n_in(0), i(0), desc(l.desc), first(true) {
     R = xcd_range(U, x, G);
   }
 
@@ -170,6 +182,10 @@ struct UnitCursor {
         n_in = min(unit, bcnt[s] - j * unit);
         base = s * ninst + j * unit;
         i = 0;
+        uidx = u;
+        // slots of a partial unit that no item fills
+        if (threadIdx.x == 0)
+          for (uint32_t q = n_in; q < unit; ++q) desc[u * unit + q] = make_uint2(0u, 0u);
         inst = bins[base];
         sub = s;
         return true;
@@ -255,7 +271,6 @@ struct ConvArgs {
                          // local copies and the interface entries before them
   uint2 *prec;           // push records (entry, time bits): instance i,
                          // submesh s use [i * n_if + ioff(s), + ni(s))
-  uint32_t *pcnt;        // n_inst x nsub: records of the last activation
   WorkList cur;          // work list of this iteration
   WorkList nxt;          // work list of the next iteration (requeues)
   unsigned long long *stats;  // activations, sweeps, element evaluations,
@@ -281,13 +296,14 @@ struct ConvArgs {
 };
 
 // Writes the interface copies lowered by this activation (bits in tch) as
-// push records into the item's slot range and their count into pcnt; the
-// entries are found through the interface list (if_l).  Every thread of the
-// workgroup calls it; cnt is one LDS word.
+// push records into the item's slot range and, when there are any, the
+// item's push descriptor into its list; the entries are found through the
+// interface list (if_l).  Every thread of the workgroup calls it; cnt is one
+// LDS word.
 template <int BS>
 __device__ inline void push_lowered_scan(const ConvArgs &a, const SubInfo &si, uint32_t inst,
-                                         size_t entry, const uint32_t *tch, const float *phi_l,
-                                         uint32_t *cnt) {
+                                         size_t entry, uint32_t dslot, const uint32_t *tch,
+                                         const float *phi_l, uint32_t *cnt) {
   const uint32_t tid = threadIdx.x, lane = tid & 63;
   if (tid == 0) *cnt = 0u;
   __syncthreads();
@@ -308,7 +324,7 @@ __device__ inline void push_lowered_scan(const ConvArgs &a, const SubInfo &si, u
     if (p) rec[wb + __popcll(m & lanes_below())] = make_uint2(si.ioff + j, __float_as_uint(phi_l[l]));
   }
   __syncthreads();
-  if (tid == 0) a.pcnt[entry] = *cnt;
+  if (tid == 0) push_descriptor(a.cur, dslot, entry, *cnt);
 }
 
 // One workgroup per work item (instance, submesh).  STAGED copies the
@@ -356,7 +372,7 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
     if (a.wake[entry] > W) {
       if (tid == 0) {
         requeue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
-        a.pcnt[entry] = 0u;
+        push_descriptor(a.cur, cursor.uidx * a.unit + cursor.i, entry, 0u);
       }
       continue;
     }
@@ -603,7 +619,7 @@ __global__ __launch_bounds__(BS) void k_conv_compute(ConvArgs a) {
         atomicMin(&misc[1], __float_as_uint(phi_l[l]));
       }
     }
-    push_lowered_scan<BS>(a, si, inst, entry, tch, phi_l, &misc[6]);
+    push_lowered_scan<BS>(a, si, inst, entry, cursor.uidx * a.unit + cursor.i, tch, phi_l, &misc[6]);
     __syncthreads();
     if (tid == 0) {
       const float wk = __uint_as_float(misc[1]);
@@ -744,7 +760,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
     if (a.wake[entry] > W) {
       if (tid == 0) {
         requeue(a.nxt, a.act, inst, s, a.nsub, a.ninst);
-        a.pcnt[entry] = 0u;
+        push_descriptor(a.cur, cursor.uidx * a.unit + cursor.i, entry, 0u);
       }
       continue;
     }
@@ -1111,7 +1127,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
     if (lane == 0 && dmin != 0xffffffffu) atomicMin(&misc[5], dmin);
     __syncthreads();
     if (tid == 0) {
-      a.pcnt[entry] = misc[6];
+      push_descriptor(a.cur, cursor.uidx * a.unit + cursor.i, entry, misc[6]);
       abytes += 32ull * misc[3] + 10ull * nn + 12ull * si.ni;
       const float wk = __uint_as_float(misc[5]);
       a.wake[entry] = wk;
@@ -1135,7 +1151,6 @@ struct SyncArgs {
   const uint4 *if_fo;  // per entry: first other copy, its submesh, then [z, w) in if_os
   const uint2 *if_os;  // per other copy: (copy, its submesh)
   const uint2 *prec;
-  const uint32_t *pcnt;
   float *phi;
   uint8_t *dirty;
   uint32_t *act;
@@ -1145,24 +1160,20 @@ struct SyncArgs {
   uint32_t nsub, ncopy, ninst, n_if, unit, maxch;
 };
 
-// interface push: one wave per item (instance, submesh) of the iteration's
-// work units, a lane per push record; min over copies, order independent.
-// The kernel waits on memory, so it runs on its own full-occupancy grid.
+// interface push: one wave per item slot of the iteration's work units (its
+// push descriptor), a lane per push record;
+// min over copies, order independent.  The kernel waits on memory, so it
+// runs on its own full-occupancy grid.
 __global__ __launch_bounds__(256) void k_conv_sync(SyncArgs a) {
   // the compute kernel of this iteration has finished reading the counter
   if (blockIdx.x == 0 && threadIdx.x == 0) *a.iter += 1u;
   const uint32_t lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
   const uint32_t NI = a.cur.cnt[0] * a.unit, nw = gridDim.x * wpb;
   for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < NI; w += nw) {
-    const uint32_t u = w / a.unit, q = w - u * a.unit;
-    const uint32_t pu = a.cur.plan[u];
-    const uint32_t s = pu / a.maxch, ch = pu - s * a.maxch;
-    const uint32_t n_in = min(a.unit, a.cur.bcnt[s] - ch * a.unit);
-    const uint32_t ioff = a.sub[s].ioff;
-    if (q < n_in) {
-      const uint32_t inst = a.cur.bins[(size_t)s * a.ninst + ch * a.unit + q];
-      const uint32_t n = a.pcnt[(size_t)inst * a.nsub + s];
-      const uint2 *rec = a.prec + (size_t)inst * a.n_if + ioff;
+    const uint2 d = a.cur.desc[w];
+    const uint32_t inst = d.x / a.nsub, s = d.x - inst * a.nsub, n = d.y;
+    if (n) {
+      const uint2 *rec = a.prec + (size_t)inst * a.n_if + a.sub[s].ioff;
       const size_t ib = (size_t)inst * a.ncopy;
       for (uint32_t r = lane; r < n; r += 64) {
         const uint2 e = rec[r];
@@ -1652,7 +1663,6 @@ struct fim_ctx {
   float *d_winst = nullptr;     // n_inst: time window per iteration (us), 0 = off
   uint32_t *d_flag = nullptr;   // n_inst: scratch for instance queries / lists
   uint2 *d_prec = nullptr;  // push records: n_inst x (interface entries)
-  uint32_t *d_pcnt = nullptr;
   uint2 *d_ifm = nullptr;
   uint32_t wps = 1;
   WorkList wl[3] = {};
@@ -2075,12 +2085,13 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
           c->d_if_fo = dupload(fo, ow);
         }
         const size_t ncs = (size_t)c->ninst * c->ncopy, nss = (size_t)c->ninst * c->nsub;
+        // push descriptors name an item by instance * nsub + submesh in 32 bits
+        if (nss >= (1ull << 32)) throw Error(FIM_ELIMIT, "instances x submeshes exceed 2^32");
         c->d_phi = dalloc<float>(ncs, ow);
         c->d_dirty = dalloc<uint8_t>(ncs, ow);
         {
           // push records: one slot per (instance, interface entry)
           c->d_prec = dalloc<uint2>((size_t)c->ninst * p.if_l.size(), ow);
-          c->d_pcnt = dalloc<uint32_t>(nss, ow);
           c->d_start = dalloc<uint32_t>(c->ninst, ow);
           c->d_winst = dupload(std::vector<float>(c->ninst, (float)std::max(0.0, o->window)), ow);
           c->d_flag = dalloc<uint32_t>(c->ninst, ow);
@@ -2117,6 +2128,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
           c->wl[k].bins = dalloc<uint32_t>(nss, ow);
           c->wl[k].plan = dalloc<uint32_t>((size_t)c->nsub * c->maxch, ow);
           c->wl[k].cnt = c->d_cnt + 16 * k;
+          c->wl[k].desc = dalloc<uint2>((size_t)c->nsub * c->maxch * c->unit, ow);
         }
         c->d_rscale = dalloc<float>(4 * (size_t)c->ninst, ow);
         c->d_tmpf = dalloc<float>(c->nn, ow);
@@ -2144,13 +2156,13 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         c->lds_compat = 0;
         c->ca = ConvArgs{c->d_sub, c->d_e2n, c->d_geo32, c->d_hgt, c->d_ereg, c->d_rscale, c->d_phi,
                          c->d_dirty, c->d_act, c->d_wake, c->d_if_l, c->d_ifm, c->d_prec,
-                         c->d_pcnt, WorkList{}, WorkList{}, c->d_stats, c->d_cnt + 48, c->d_start,
+                         WorkList{}, WorkList{}, c->d_stats, c->d_cnt + 48, c->d_start,
                          c->d_n2e_dsp, c->d_n2e_le, c->d_erec, c->d_adj_ptr, c->d_adj,
                          c->nsub, c->ncopy, c->ninst, (float)(o->eps_rel > 0 ? o->eps_rel : 1e-6),
                          c->d_winst, o->local_iters, c->unit, c->maxch, (uint32_t)p.if_l.size(),
                          c->wps, elcap, o_phin, o_flg, o_tch, o_dfr, o_e2n, o_geo, o_list, o_vlist, o_misc,
                          o_wrt, o_ifc};
-        c->sa = SyncArgs{c->d_sub, c->d_if_fo, c->d_if_os, c->d_prec, c->d_pcnt,
+        c->sa = SyncArgs{c->d_sub, c->d_if_fo, c->d_if_os, c->d_prec,
                          c->d_phi, c->d_dirty, c->d_act, c->d_wake, WorkList{}, WorkList{},
                          c->d_cnt + 48, c->nsub, c->ncopy, c->ninst, (uint32_t)p.if_l.size(),
                          c->unit, c->maxch};
