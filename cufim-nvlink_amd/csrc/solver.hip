@@ -1451,9 +1451,13 @@ __global__ __launch_bounds__(BS) void k_compat_compute(CompatArgs a) {
     const uint32_t nn = si.nn;
     // update_states (cuda_fim.cu:1790-1845), applied to every submesh
     uint32_t cnt = 0;
+    double p0 = 0.0;  // the copy as loaded (one copy per thread when nn <= BS)
+    int8_t st0 = 0;
     for (uint32_t l = tid; l < nn; l += BS) {
       double p = a.phi[si.coff + l];
       int8_t st = a.st[si.coff + l];
+      p0 = p;
+      st0 = st;
       if (st == FIM_NARROW) {
         ++cnt;
       } else if (st == FIM_NEXT) {
@@ -1471,6 +1475,7 @@ __global__ __launch_bounds__(BS) void k_compat_compute(CompatArgs a) {
       st_l[l] = st;
     }
     uint32_t active = block_sum(cnt, misc);
+    const bool idle = active == 0;
     // the submesh's (vertex, element) slots, contiguous in the copy-major CSR
     const uint32_t k0 = a.n2e_dsp[si.coff], k1 = a.n2e_dsp[si.coff + nn];
     unsigned long long *tmin = reinterpret_cast<unsigned long long *>(tphi);
@@ -1540,6 +1545,17 @@ __global__ __launch_bounds__(BS) void k_compat_compute(CompatArgs a) {
         }
       }
       active = block_sum(cnt, misc);
+    }
+    if (idle && nn <= BS) {
+      // nothing was narrow: only update_states' time-driven transitions can
+      // have changed a copy, and nothing is pending (narrow and next copies
+      // are what `active` counted)
+      for (uint32_t l = tid; l < nn; l += BS)
+        if (phi_l[l] != p0 || st_l[l] != st0) {
+          a.phi[si.coff + l] = phi_l[l];
+          a.st[si.coff + l] = st_l[l];
+        }
+      continue;
     }
     uint32_t pend = 0;
     for (uint32_t l = tid; l < nn; l += BS) {
@@ -1763,7 +1779,9 @@ void set_lds_attr(fim_ctx *c) {
 }
 
 void launch_compat_compute(fim_ctx *c, const CompatArgs &a) {
-  const int grid = (int)std::min<uint32_t>(c->nsub, 4096u);
+  // 8192 workgroups: more submeshes in flight than 4096 (-4 % per window on
+  // the slab's 10,162), no further gain from one workgroup per submesh
+  const int grid = (int)std::min<uint32_t>(c->nsub, 8192u);
   switch (c->block) {
     case 64: k_compat_compute<64><<<grid, 64, c->lds_compat, c->stream>>>(a); break;
     case 128: k_compat_compute<128><<<grid, 128, c->lds_compat, c->stream>>>(a); break;
