@@ -1374,6 +1374,7 @@ struct CompatArgs {
   int8_t *st;
   unsigned long long *stats;  // sweeps, slot evaluations
   uint32_t *pending;          // copies left narrow/next after the window
+  double *nexte;              // per submesh: earliest window time its copies change
   uint32_t nsub;
   int32_t exact;
   double fw, cur, phi_start, dt, apd;
@@ -1389,6 +1390,26 @@ __device__ inline uint32_t block_sum(uint32_t v, uint32_t *slot) {
   const uint32_t r = *slot;
   __syncthreads();
   return r;
+}
+
+// FP64 bit pattern -> unsigned key of the same order (no NaN) and back
+__device__ inline unsigned long long dkey(double d) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(d);
+  return (b >> 63) ? ~b : (b | (1ull << 63));
+}
+__device__ inline double dkey_inv(unsigned long long k) {
+  return __longlong_as_double((long long)((k >> 63) ? (k & ~(1ull << 63)) : ~k));
+}
+
+// minimum over the block, valid in thread 0 only (the slot is reset by
+// thread 0 before its first barrier, so back-to-back calls need no third one)
+__device__ inline double block_min0(double v, unsigned long long *slot) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+  if (threadIdx.x == 0) *slot = ~0ull;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) atomicMin(slot, dkey(v));
+  __syncthreads();
+  return threadIdx.x == 0 ? dkey_inv(*slot) : 0.0;
 }
 
 // one (vertex, element) slot: src/eikonal/cuda_fim.h:1362-1411.  The slot's
@@ -1446,7 +1467,11 @@ __global__ __launch_bounds__(BS) void k_compat_compute(CompatArgs a) {
   uint32_t *misc = reinterpret_cast<uint32_t *>(lds + a.o_misc);
   const uint32_t tid = threadIdx.x;
   unsigned long long sweeps = 0, slots = 0;
+  unsigned long long *evslot = reinterpret_cast<unsigned long long *>(misc + 2);
   for (uint32_t s = blockIdx.x; s < a.nsub; s += gridDim.x) {
+    // a submesh with nothing narrow or next, whose frozen and tail copies all
+    // stay put at this window time, would come out of the pass unchanged
+    if (a.nexte[s] > a.cur) continue;
     const SubInfo si = a.sub[s];
     const uint32_t nn = si.nn;
     // update_states (cuda_fim.cu:1790-1845), applied to every submesh
@@ -1546,25 +1571,36 @@ __global__ __launch_bounds__(BS) void k_compat_compute(CompatArgs a) {
       }
       active = block_sum(cnt, misc);
     }
-    if (idle && nn <= BS) {
-      // nothing was narrow: only update_states' time-driven transitions can
-      // have changed a copy, and nothing is pending (narrow and next copies
-      // are what `active` counted)
-      for (uint32_t l = tid; l < nn; l += BS)
-        if (phi_l[l] != p0 || st_l[l] != st0) {
-          a.phi[si.coff + l] = phi_l[l];
-          a.st[si.coff + l] = st_l[l];
-        }
-      continue;
-    }
+    // write back; when nothing was narrow only update_states' time-driven
+    // transitions can have changed a copy, and nothing is pending (narrow and
+    // next copies are what `active` counted).  ev: the earliest window time
+    // at which update_states would change a copy (frozen: p + apd, tail:
+    // p + apd / 0.91, as update_states computes them; narrow or next: now)
+    const bool lean = idle && nn <= BS;
     uint32_t pend = 0;
+    double ev = INFINITY;
     for (uint32_t l = tid; l < nn; l += BS) {
-      a.phi[si.coff + l] = phi_l[l];
-      a.st[si.coff + l] = st_l[l];
-      pend += (st_l[l] == FIM_NARROW || st_l[l] == FIM_NEXT);
+      const double p = phi_l[l];
+      const int8_t st = st_l[l];
+      if (!lean || p != p0 || st != st0) {
+        a.phi[si.coff + l] = p;
+        a.st[si.coff + l] = st;
+      }
+      if (st == FIM_NARROW || st == FIM_NEXT) {
+        ++pend;
+        ev = -INFINITY;
+      } else if (st == FIM_FROZEN) {
+        ev = fmin(ev, p + a.apd);
+      } else if (st == FIM_TAIL) {
+        ev = fmin(ev, p + a.apd / 0.91);
+      }
     }
-    pend = block_sum(pend, misc);
-    if (tid == 0 && pend) atomicAdd(a.pending, pend);
+    if (!lean) {
+      pend = block_sum(pend, misc);
+      if (tid == 0 && pend) atomicAdd(a.pending, pend);
+    }
+    ev = block_min0(ev, evslot);
+    if (tid == 0) a.nexte[s] = ev;
   }
   if (tid == 0 && sweeps) atomicAdd(&a.stats[0], sweeps);
   for (int off = 32; off > 0; off >>= 1) slots += __shfl_xor(slots, off);
@@ -1573,7 +1609,8 @@ __global__ __launch_bounds__(BS) void k_compat_compute(CompatArgs a) {
 
 // sync_step2 over vertices with several copies
 __global__ void k_compat_sync(const uint32_t *grp_v, uint32_t ngrp, const uint32_t *g2c_dsp,
-                              const uint32_t *g2c, double *phi, int8_t *st) {
+                              const uint32_t *g2c, const uint32_t *c2s, double *phi, int8_t *st,
+                              double *nexte) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ngrp; i += gridDim.x * blockDim.x) {
     const uint32_t v = grp_v[i];
     double mn = DBL_MAX;
@@ -1586,15 +1623,18 @@ __global__ void k_compat_sync(const uint32_t *grp_v, uint32_t ngrp, const uint32
       if (mn < phi[c]) {
         phi[c] = mn;
         st[c] = FIM_NARROW;
+        nexte[c2s[c]] = -INFINITY;
       }
     }
   }
 }
 
-__global__ void k_compat_seed(const uint2 *ic, uint32_t n, double t0, double *phi, int8_t *st) {
+__global__ void k_compat_seed(const uint2 *ic, uint32_t n, double t0, const uint32_t *c2s, double *phi,
+                              int8_t *st, double *nexte) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     phi[ic[i].y] = t0;
     st[ic[i].y] = FIM_NARROW;
+    nexte[c2s[ic[i].y]] = -INFINITY;
   }
 }
 
@@ -1687,6 +1727,7 @@ struct fim_ctx {
   // compat state
   double *d_phi64 = nullptr;
   int8_t *d_st = nullptr;
+  double *d_nexte = nullptr;  // per submesh: earliest window time a copy changes
   uint32_t *d_pending = nullptr;
   // shared
   unsigned long long *d_stats = nullptr;
@@ -2205,12 +2246,13 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         c->d_phi64 = dalloc<double>(c->ncopy, ow);
         c->d_st = dalloc<int8_t>(c->ncopy, ow);
         c->d_pending = dalloc<uint32_t>(1, ow);
+        c->d_nexte = dalloc<double>(c->nsub, ow);
         c->d_tmpd = dalloc<double>(c->nn, ow);
         c->d_tmpi = dalloc<int32_t>(c->nn, ow);
         c->lds_conv = 0;
         c->xa = CompatArgs{c->d_sub, c->d_e2n, c->d_geo64, c->d_n2e_dsp, c->d_n2e_le, c->d_n2e_pos,
                            c->d_n2n_dsp, c->d_n2n_lv, c->d_phi64, c->d_st, c->d_stats,
-                           c->d_pending, c->nsub, o->local == FIM_LOCAL_EXACT ? 1 : 0,
+                           c->d_pending, c->d_nexte, c->nsub, o->local == FIM_LOCAL_EXACT ? 1 : 0,
                            0, 0, 0, 0, o->apd > 0 ? o->apd : 3e5,
                            o_tphi, o_st, o_tst, o_snap, o_misc};
       }
@@ -2247,6 +2289,7 @@ int fim_reset(fim_ctx *c) {
     } else {
       k_fill<double><<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_phi64, c->ncopy, DBL_MAX);
       k_fill<int8_t><<<blocks_for(c->ncopy, 256), 256, 0, c->stream>>>(c->d_st, c->ncopy, (int8_t)FIM_FAR);
+      k_fill<double><<<blocks_for(c->nsub, 256), 256, 0, c->stream>>>(c->d_nexte, c->nsub, -INFINITY);
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -2281,7 +2324,8 @@ void seed_pairs(fim_ctx *c, const std::vector<uint2> &ic, double t0) {
                                           c->d_dirty, c->d_act, c->d_wake, c->wl[0], c->nsub,
                                           c->ncopy, c->ninst);
   } else {
-    k_compat_seed<<<g, 256, 0, c->stream>>>(c->d_seed, (uint32_t)ic.size(), t0, c->d_phi64, c->d_st);
+    k_compat_seed<<<g, 256, 0, c->stream>>>(c->d_seed, (uint32_t)ic.size(), t0, c->d_c2s, c->d_phi64,
+                                            c->d_st, c->d_nexte);
   }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -2374,8 +2418,8 @@ int fim_step(fim_ctx *c, double phi_start, double cur_time, double dt, double fr
     HIPCHECK(hipSetDevice(c->dev));
     HIPCHECK(hipEventRecord(c->ev0, c->stream));
     if (c->ngrp)
-      k_compat_sync<<<blocks_for(c->ngrp, 256), 256, 0, c->stream>>>(c->d_grp, c->ngrp, c->d_g2c_dsp,
-                                                                      c->d_g2c, c->d_phi64, c->d_st);
+      k_compat_sync<<<blocks_for(c->ngrp, 256), 256, 0, c->stream>>>(
+          c->d_grp, c->ngrp, c->d_g2c_dsp, c->d_g2c, c->d_c2s, c->d_phi64, c->d_st, c->d_nexte);
     k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_pending, 1, 0u);
     CompatArgs a = c->xa;
     a.fw = front_width;
