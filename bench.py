@@ -231,6 +231,9 @@ def main():
     ap.add_argument("--windows-in", default="",
                     help="counter runs: replay saved per-instance windows for the accounting batch "
                          "without the surrogate, print the roofline fields and stop")
+    ap.add_argument("--nn-only", action="store_true",
+                    help="counter runs: build the surrogate's dataset (--nn-maps) and train "
+                         "(--nn-steps), print the training rate and stop")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -292,6 +295,11 @@ def main():
         done += k
         log(f"[bench] surrogate training {done}/{args.nn_steps} steps, loss {loss:.3e}")
     t_train = time.perf_counter() - t0
+    if args.nn_only:
+        nn.close()
+        print(json.dumps({"surrogate_only": True, "steps": args.nn_steps, "batch": args.nn_batch,
+                          "ms_per_step": 1e3 * t_train / max(args.nn_steps, 1), "loss": loss}), flush=True)
+        return
     sample = WS.window_sample(mesh, k=args.sample, seed=args.seed)
 
     # ---- accounting: event-timed compute launches of one scheduled batch ----
