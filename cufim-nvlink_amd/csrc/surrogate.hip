@@ -197,9 +197,10 @@ struct ChainArgs {
   float inv_b, t_scale, delta;
 };
 
-// The whole network for one tile of 128 rows, activations in LDS between
-// layers.  Eight waves: rows 64 rg .. +63 (rg = wave / 4), columns 64 fg ..
-// +63 (fg = wave % 4) of every product.
+// The whole network for one tile of TR = 64 RG rows, activations in LDS
+// between layers.  Four waves per row group: rows 64 rg .. +63 (rg = wave /
+// 4; RG = 1, so one group of four waves), columns 64 fg .. +63 (fg = wave % 4)
+// of every product, as 2 x 2 accumulators of 32 x 32.
 //   forward: Z_l = H_{l-1} W_l^T + b_l, H_l = SiLU(Z_l) (BF16 in LDS; Z_l to
 //     HBM in the k-step layout when training);
 //   output: y = H_5 . w_out + b_out (inference: t = y t_scale + delta);
