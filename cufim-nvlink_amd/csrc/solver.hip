@@ -1717,6 +1717,12 @@ struct fim_ctx {
   uint32_t *d_act = nullptr, *d_cnt = nullptr;
   uint32_t *d_start = nullptr;  // n_inst: iteration each instance was (re)seeded at
   float *d_winst = nullptr;     // n_inst: time window per iteration (us), 0 = off
+  // per-instance region factors and windows as set on the host
+  // (fim_set_scaling, fim_set_window), uploaded together before the next
+  // launch that reads them: setting up a batch costs no device round trip
+  // per instance
+  std::vector<float> h_rscale, h_winst;
+  bool params_dirty = false;
   uint32_t *d_flag = nullptr;   // n_inst: scratch for instance queries / lists
   uint2 *d_prec = nullptr;  // push records: n_inst x (interface entries)
   uint2 *d_ifm = nullptr;
@@ -1968,9 +1974,22 @@ uint32_t *noalloc_scratch(int dev) {
   return ptr[dev];
 }
 
+// uploads the per-instance factors and windows set since the last upload
+// (the device must be current)
+void flush_params(fim_ctx *c) {
+  if (!c->params_dirty) return;
+  HIPCHECK(hipMemcpyAsync(c->d_rscale, c->h_rscale.data(), c->h_rscale.size() * sizeof(float),
+                          hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->d_winst, c->h_winst.data(), c->h_winst.size() * sizeof(float),
+                          hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->params_dirty = false;
+}
+
 // support check of one instance with `best` (nn words of device memory) as
 // the per-vertex candidate minimum
 double support_into(fim_ctx *c, int32_t inst, uint32_t *best) {
+  flush_params(c);
   k_fill<uint32_t><<<blocks_for(c->nn, 256), 256, 0, c->stream>>>(best, c->nn, 0x7f800000u);
   k_conv_support<<<(int)std::min<uint32_t>(c->nsub, 4096u), 256, 0, c->stream>>>(
       c->d_sub, c->nsub, c->d_e2n, c->d_geo32, c->d_ereg, c->d_rscale, c->d_phi, c->d_c2g,
@@ -2152,7 +2171,8 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
           // push records: one slot per (instance, interface entry)
           c->d_prec = dalloc<uint2>((size_t)c->ninst * p.if_l.size(), ow);
           c->d_start = dalloc<uint32_t>(c->ninst, ow);
-          c->d_winst = dupload(std::vector<float>(c->ninst, (float)std::max(0.0, o->window)), ow);
+          c->h_winst.assign(c->ninst, (float)std::max(0.0, o->window));
+          c->d_winst = dupload(c->h_winst, ow);
           c->d_flag = dalloc<uint32_t>(c->ninst, ow);
           HIPCHECK(hipMemset(c->d_start, 0, sizeof(uint32_t) * c->ninst));
           // per submesh: interface-copy bits of each 32 local copies and the
@@ -2192,6 +2212,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
         c->d_rscale = dalloc<float>(4 * (size_t)c->ninst, ow);
         c->d_tmpf = dalloc<float>(c->nn, ow);
         k_fill<float><<<blocks_for(4 * (size_t)c->ninst, 256), 256, 0, c->stream>>>(c->d_rscale, 4 * (size_t)c->ninst, 1.0f);
+        c->h_rscale.assign(4 * (size_t)c->ninst, 1.0f);
         int ncu = 256;
         HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
         // one resident wave of workgroups: registers, LDS and wave slots.
@@ -2373,9 +2394,8 @@ int fim_set_scaling(fim_ctx *c, int32_t inst, const float scale[4]) {
       if (!(scale[k] > 0.0f)) throw Error(FIM_EINVAL, "scalings must be positive");
       m[k] = 1.0f / (scale[k] * scale[k]);  // both speeds scaled: M scales by 1/s^2
     }
-    HIPCHECK(hipSetDevice(c->dev));
-    HIPCHECK(hipMemcpyAsync(c->d_rscale + 4 * (size_t)inst, m, sizeof(m), hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    std::memcpy(&c->h_rscale[4 * (size_t)inst], m, sizeof(m));
+    c->params_dirty = true;
   });
 }
 
@@ -2385,10 +2405,8 @@ int fim_set_window(fim_ctx *c, int32_t inst, double window) {
     if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "time windows need converge mode");
     check_inst(c, inst);
     if (!(window >= 0.0) || !(window < 1e30)) throw Error(FIM_EINVAL, "window must be >= 0 and finite");
-    const float w = (float)window;
-    HIPCHECK(hipSetDevice(c->dev));
-    HIPCHECK(hipMemcpyAsync(c->d_winst + inst, &w, sizeof(float), hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->h_winst[inst] = (float)window;
+    c->params_dirty = true;
   });
 }
 
@@ -2444,6 +2462,7 @@ namespace {
 // last reset_stats call.
 uint32_t run_iterations(fim_ctx *c, int32_t max_iters, bool reset_stats) {
   HIPCHECK(hipSetDevice(c->dev));
+  flush_params(c);
   // profile mode launches the same sequence of kernels without the graph
   // (events cannot be timed inside a graph) and times every compute launch
   if (c->o.profile && c->kev.empty()) {
@@ -2614,6 +2633,7 @@ int fim_residual(fim_ctx *c, int32_t inst, double *res) {
     if (c->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "residual needs converge mode");
     check_inst(c, inst);
     HIPCHECK(hipSetDevice(c->dev));
+    flush_params(c);
     k_fill<uint32_t><<<1, 64, 0, c->stream>>>(c->d_tmpu, 1, 0u);
     k_conv_residual<<<(int)std::min<uint32_t>(c->nsub, 4096u), 256, 0, c->stream>>>(
         c->d_sub, c->nsub, c->d_e2n, c->d_geo32, c->d_ereg, c->d_rscale, c->d_phi,
