@@ -231,6 +231,9 @@ def main():
     ap.add_argument("--windows-in", default="",
                     help="counter runs: replay saved per-instance windows for the accounting batch "
                          "without the surrogate, print the roofline fields and stop")
+    ap.add_argument("--contexts", type=int, default=2,
+                    help="solver contexts the held-out batch is split over (fim_run_multi: their "
+                         "streams overlap, one context's launch tails filled by another's work)")
     ap.add_argument("--nn-only", action="store_true",
                     help="counter runs: build the surrogate's dataset (--nn-maps) and train "
                          "(--nn-steps), print the training rate and stop")
@@ -318,8 +321,13 @@ def main():
         f"{pre['compute_launches']}, {st['compute_ms']:.1f} ms of compute kernel")
 
     # ---- headline: surrogate-scheduled batches, graph replay ----------------
-    s = F.Solver(pack, n_instances=args.test, block=args.block, unit=args.unit,
-                 local_iters=args.local_iters, window=args.window)
+    # the batch split over `contexts` solver contexts that run together; every
+    # map is the one a single context gives (checked below against the
+    # accounting batch's first map)
+    nctx = max(1, min(args.contexts, args.test))
+    cut = np.linspace(0, args.test, nctx + 1).astype(int)
+    s = F.SolverGroup([F.Solver(pack, n_instances=int(cut[k + 1] - cut[k]), block=args.block, unit=args.unit,
+                                local_iters=args.local_iters, window=args.window) for k in range(nctx)])
     for i in range(args.warmup):
         WS.scheduled_batch(nn, s, test, sample, window=args.window, quantile=args.quantile)
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
@@ -478,12 +486,14 @@ def main():
         "config": {
             "workload": "anisotropic slab 160x160x13 hexes x 6 = 1,996,800 tets / 362,894 vertices, fibres "
                         "-60..+60 deg, v_f 0.6 / v_c 0.24 m/s, four quadrant regions; "
-                        f"{args.test} held-out solves per step as one batch (1-3 pacing sites on z = 0, "
+                        f"{args.test} held-out solves per step as one batch split over {nctx} solver "
+                        "contexts run together (1-3 pacing sites on z = 0, "
                         "regional speed factors U[0.5, 1.5]), each instance's time window from the "
                         "surrogate's predicted latest activation (inference inside the timed region), "
                         "batched FIM to eps 1e-6, graph replay; surrogate trained in the run on "
                         f"{args.nn_maps} batched-FIM maps (set-up, not timed)",
             "maps_per_step": args.test,
+            "contexts": nctx,
             "partition": f"recursive bisection, <= {args.max_nn} vertex copies per submesh, "
                          f"{info['nsub']} submeshes",
             "max_nn": args.max_nn,

@@ -36,6 +36,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -1748,6 +1749,7 @@ struct fim_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evc = nullptr;  // end of an iteration chunk's read-back
   std::vector<hipEvent_t> kev;  // profile: events around each captured compute launch
   ConvArgs ca{};
   SyncArgs sa{};
@@ -1899,6 +1901,7 @@ void destroy_ctx(fim_ctx *c) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->evc) (void)hipEventDestroy(c->evc);
   for (hipEvent_t e : c->kev)
     if (e) (void)hipEventDestroy(e);
   for (void *p : c->owned) (void)hipFree(p);
@@ -2031,6 +2034,7 @@ int fim_create(const fim_pack *pk, const fim_opts *o, fim_ctx **out) {
       HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       HIPCHECK(hipEventCreate(&c->ev0));
       HIPCHECK(hipEventCreate(&c->ev1));
+      HIPCHECK(hipEventCreateWithFlags(&c->evc, hipEventDisableTiming));
       if (o->mode != FIM_MODE_CONVERGE && o->mode != FIM_MODE_COMPAT)
         throw Error(FIM_EINVAL, "unknown mode");
       c->block = o->block ? o->block : (o->mode == FIM_MODE_CONVERGE ? 512 : 256);
@@ -2456,11 +2460,19 @@ int fim_step(fim_ctx *c, double phi_start, double cur_time, double dt, double fr
 }
 
 namespace {
-// Runs global iterations in graph-sized chunks until no work is queued or
-// max_iters (rounded up to whole chunks) have run; returns the queued units
-// left (0 = every instance converged).  Device counters accumulate from the
-// last reset_stats call.
-uint32_t run_iterations(fim_ctx *c, int32_t max_iters, bool reset_stats) {
+// The loop of global iterations in phases -- begin, a chunk of graph_iters
+// iterations with the read-back of the queued units, collect, end -- so that
+// one host thread can also drive several contexts at once (fim_run_multi),
+// whose streams then overlap on the device.  Device counters accumulate from
+// the last begin with reset_stats.
+struct RunState {
+  int32_t max_iters = 0, done = 0;
+  uint32_t left = 1;
+  double kms = 0.0;
+  int64_t klaunch = 0;
+};
+
+void run_begin(fim_ctx *c, int32_t max_iters, bool reset_stats, RunState &r) {
   HIPCHECK(hipSetDevice(c->dev));
   flush_params(c);
   // profile mode launches the same sequence of kernels without the graph
@@ -2478,30 +2490,41 @@ uint32_t run_iterations(fim_ctx *c, int32_t max_iters, bool reset_stats) {
   HIPCHECK(hipEventRecord(c->ev0, c->stream));
   // units of the list the seeds (or an earlier run) filled
   launch_plan(c, 0, -1);
-  int32_t done = 0;
-  uint32_t left = 1;
-  double kms = 0.0;
-  int64_t klaunch = 0;
-  while (done < max_iters) {
-    if (c->o.profile || c->direct) {
-      for (int i = 0; i < c->graph_iters; ++i) enqueue_iteration(c, i, c->o.profile != 0);
-    } else {
-      HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
-    }
-    done += c->graph_iters;
-    HIPCHECK(hipMemcpyAsync(c->h_pin, c->d_cnt, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
-    if (c->o.profile) {
-      for (int j = 0; j < c->graph_iters; ++j) {
-        float ms = 0;
-        HIPCHECK(hipEventElapsedTime(&ms, c->kev[2 * j], c->kev[2 * j + 1]));
-        kms += ms;
-        ++klaunch;
-      }
-    }
-    left = c->h_pin[0];
-    if (left == 0) break;
+  r = RunState{};
+  r.max_iters = max_iters;
+}
+
+// enqueues graph_iters iterations, the read-back of the queued units and the
+// chunk's completion event
+void run_chunk(fim_ctx *c, RunState &r) {
+  HIPCHECK(hipSetDevice(c->dev));
+  if (c->o.profile || c->direct) {
+    for (int i = 0; i < c->graph_iters; ++i) enqueue_iteration(c, i, c->o.profile != 0);
+  } else {
+    HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
   }
+  r.done += c->graph_iters;
+  HIPCHECK(hipMemcpyAsync(c->h_pin, c->d_cnt, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipEventRecord(c->evc, c->stream));
+}
+
+// after the chunk's event has completed: launch times (profile mode) and
+// queued units; true while more iterations are to run
+bool run_collect(fim_ctx *c, RunState &r) {
+  if (c->o.profile) {
+    for (int j = 0; j < c->graph_iters; ++j) {
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, c->kev[2 * j], c->kev[2 * j + 1]));
+      r.kms += ms;
+      ++r.klaunch;
+    }
+  }
+  r.left = c->h_pin[0];
+  return r.left != 0 && r.done < r.max_iters;
+}
+
+void run_end(fim_ctx *c, RunState &r) {
+  HIPCHECK(hipSetDevice(c->dev));
   HIPCHECK(hipEventRecord(c->ev1, c->stream));
   unsigned long long st[8];
   HIPCHECK(hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
@@ -2515,9 +2538,22 @@ uint32_t run_iterations(fim_ctx *c, int32_t max_iters, bool reset_stats) {
   c->stats.global_iters = (int64_t)st[3];
   c->stats.alg_bytes = (int64_t)st[4];
   c->stats.vertex_updates = (int64_t)st[5];
-  c->stats.compute_ms += kms;
-  c->stats.compute_launches += klaunch;
-  return left;
+  c->stats.compute_ms += r.kms;
+  c->stats.compute_launches += r.klaunch;
+}
+
+// Runs global iterations in graph-sized chunks until no work is queued or
+// max_iters (rounded up to whole chunks) have run; returns the queued units
+// left (0 = every instance converged).
+uint32_t run_iterations(fim_ctx *c, int32_t max_iters, bool reset_stats) {
+  RunState r;
+  run_begin(c, max_iters, reset_stats, r);
+  do {
+    run_chunk(c, r);
+    HIPCHECK(hipEventSynchronize(c->evc));
+  } while (run_collect(c, r));
+  run_end(c, r);
+  return r.left;
 }
 
 std::vector<uint32_t> checked_instances(const fim_ctx *c, int32_t n, const int32_t *insts) {
@@ -2549,6 +2585,53 @@ int fim_advance(fim_ctx *c, int32_t iters, int32_t reset_stats, int32_t *left) {
     if (iters <= 0) throw Error(FIM_EINVAL, "iters must be positive");
     const uint32_t l = run_iterations(c, iters, reset_stats != 0);
     if (left) *left = (int32_t)l;
+  });
+}
+
+int fim_run_multi(fim_ctx *const *cs, int32_t n, int32_t max_iters, int32_t *iters) {
+  return guarded([&] {
+    if (!cs || n <= 0) throw Error(FIM_EINVAL, "bad context list");
+    if (max_iters <= 0) throw Error(FIM_EINVAL, "max_iters must be positive");
+    for (int32_t i = 0; i < n; ++i) {
+      if (!cs[i]) throw Error(FIM_EINVAL, "null context");
+      if (cs[i]->o.mode != FIM_MODE_CONVERGE) throw Error(FIM_ESTATE, "fim_run_multi needs converge mode");
+      for (int32_t j = 0; j < i; ++j)
+        if (cs[j] == cs[i]) throw Error(FIM_EINVAL, "context listed twice");
+    }
+    std::vector<RunState> rs(n);
+    std::vector<uint8_t> running(n, 1);
+    for (int32_t i = 0; i < n; ++i) {
+      run_begin(cs[i], max_iters, true, rs[i]);
+      run_chunk(cs[i], rs[i]);
+    }
+    // the host polls the chunks' events and keeps every stream fed; the
+    // streams' kernels share the device, so one context's launch tail is
+    // filled by another's work
+    for (int32_t active = n; active > 0;) {
+      bool progress = false;
+      for (int32_t i = 0; i < n; ++i) {
+        if (!running[i]) continue;
+        HIPCHECK(hipSetDevice(cs[i]->dev));
+        const hipError_t q = hipEventQuery(cs[i]->evc);
+        if (q == hipErrorNotReady) continue;
+        HIPCHECK(q);
+        progress = true;
+        if (run_collect(cs[i], rs[i])) {
+          run_chunk(cs[i], rs[i]);
+        } else {
+          run_end(cs[i], rs[i]);
+          running[i] = 0;
+          --active;
+        }
+      }
+      if (!progress) std::this_thread::yield();
+    }
+    bool capped = false;
+    for (int32_t i = 0; i < n; ++i) {
+      if (iters) iters[i] = (int32_t)cs[i]->stats.global_iters;
+      capped = capped || rs[i].left != 0;
+    }
+    if (capped) throw Error(FIM_ENOCONV, "iteration cap reached before convergence");
   });
 }
 

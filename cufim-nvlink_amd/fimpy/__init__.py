@@ -112,6 +112,7 @@ SIGNATURES = {
     "fim_seed": (C.c_int, [_vp, C.c_int32, _pi64, C.c_int64, C.c_double]),
     "fim_seed_batch": (C.c_int, [_vp, C.c_int32, _pi32, _pi64, _pi64, C.c_double]),
     "fim_advance": (C.c_int, [_vp, C.c_int32, C.c_int32, _pi32]),
+    "fim_run_multi": (C.c_int, [C.POINTER(_vp), C.c_int32, C.c_int32, _pi32]),
     "fim_active": (C.c_int, [_vp, _pi32]),
     "fim_recycle": (C.c_int, [_vp, C.c_int32, _pi32]),
     "fim_set_scaling": (C.c_int, [_vp, C.c_int32, _pf]),
@@ -431,6 +432,68 @@ def load_plan(path, opts=None):
                            vtx=vtx[k:k + cnt[i]].copy()))
         k += cnt[i]
     return {k: getattr(o, k) for k, _ in CfgOptions._fields_}, events
+
+
+def run_many(solvers, max_iters=100000):
+    """fim_run for several converge-mode contexts at once (their streams
+    overlap on the device); returns each one's global iterations."""
+    n = len(solvers)
+    hs = (_vp * n)(*[s.h for s in solvers])
+    it = (C.c_int32 * n)()
+    _check(lib().fim_run_multi(hs, n, max_iters, it))
+    return list(it)
+
+
+class SolverGroup:
+    """Converge-mode contexts solved together (fim_run_multi): instance i of
+    the group is instance i - off[k] of context k.  Splitting a batch over two
+    contexts lets one context's work fill the other's launch tails; the maps
+    are the same as from one context."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        assert self.solvers
+        self.off = np.cumsum([0] + [s.n_instances for s in self.solvers]).astype(np.int64)
+        self.window = self.solvers[0].window
+
+    @property
+    def n_instances(self):
+        return int(self.off[-1])
+
+    def parts(self, n):
+        """(context, first, end) of the group's instances 0..n-1, per context"""
+        return [(s, int(self.off[k]), int(min(self.off[k + 1], n)))
+                for k, s in enumerate(self.solvers) if self.off[k] < n]
+
+    def _at(self, inst):
+        if not 0 <= inst < self.n_instances:
+            raise FimError(-1, f"instance {inst} out of range")
+        k = int(np.searchsorted(self.off, inst, side="right")) - 1
+        return self.solvers[k], int(inst - self.off[k])
+
+    def download(self, inst=0):
+        s, j = self._at(inst)
+        return s.download(inst=j)
+
+    def residual(self, inst=0):
+        s, j = self._at(inst)
+        return s.residual(inst=j)
+
+    def run(self, max_iters=100000):
+        return max(run_many(self.solvers, max_iters))
+
+    def stats(self):
+        """counters summed over the contexts; global iterations and device
+        time of the longest"""
+        sts = [s.stats() for s in self.solvers]
+        out = {k: sum(st[k] for st in sts) for k in sts[0]}
+        out["global_iters"] = max(st["global_iters"] for st in sts)
+        out["last_ms"] = max(st["last_ms"] for st in sts)
+        return out
+
+    def close(self):
+        for s in self.solvers:
+            s.close()
 
 
 def run_reference(mesh, part, plan, out, reps=1, cfg="data/default.cfg", device=0):

@@ -21,7 +21,7 @@ import time
 
 import numpy as np
 
-from . import PART_RCB, Pack, Solver, Surrogate  # noqa: F401
+from . import PART_RCB, Pack, Solver, SolverGroup, Surrogate  # noqa: F401
 from .workloads import pacing_sites, splitmix64
 
 
@@ -69,19 +69,33 @@ def scheduled_windows(tmax, window, quantile=0.75, floor=250.0):
     return np.maximum(floor, tmax / k)
 
 
-def windowed_batch(solver: Solver, solves, wins):
-    """Batched solve of len(solves) instances, instance i at window wins[i];
-    returns the solver stats."""
-    solver.reset()
-    for i, (sites, sc) in enumerate(solves):
-        solver.set_scaling(sc, inst=i)
-        solver.set_window(float(wins[i]), inst=i)
-    solver.seed_batch([s for s, _ in solves], 0.0)
+def _contexts(solver, n):
+    """(context, first, end) parts of instances 0..n-1: a SolverGroup's
+    contexts, or the one context with every instance"""
+    return solver.parts(n) if isinstance(solver, SolverGroup) else [(solver, 0, n)]
+
+
+def _batch(solver, solves, wins):
+    """Set-up and solve of len(solves) instances, instance i at window
+    wins[i]: every context of a SolverGroup takes its own instances and the
+    contexts run together."""
+    for ctx, a, b in _contexts(solver, len(solves)):
+        ctx.reset()
+        for i in range(a, b):
+            ctx.set_scaling(solves[i][1], inst=i - a)
+            ctx.set_window(float(wins[i]), inst=i - a)
+        ctx.seed_batch([s for s, _ in solves[a:b]], 0.0)
     solver.run(max_iters=1_000_000)
     return solver.stats()
 
 
-def scheduled_batch(nn: Surrogate, solver: Solver, solves, sample, window=2000.0, quantile=0.75,
+def windowed_batch(solver, solves, wins):
+    """Batched solve of len(solves) instances, instance i at window wins[i],
+    on a Solver or a SolverGroup; returns the solver stats."""
+    return _batch(solver, solves, wins)
+
+
+def scheduled_batch(nn: Surrogate, solver, solves, sample, window=2000.0, quantile=0.75,
                     wins_out=None):
     """Surrogate-scheduled batched solve of len(solves) instances: predicted
     latest activation per solve (one device call over the vertex sample),
@@ -104,13 +118,7 @@ def cold_batch(solver, solves, window=None):
     window (the context's, or `window` when given); returns the solver
     stats."""
     w = solver.window if window is None else window
-    solver.reset()
-    for i, (sites, sc) in enumerate(solves):
-        solver.set_scaling(sc, inst=i)
-        solver.set_window(w, inst=i)
-    solver.seed_batch([s for s, _ in solves], 0.0)
-    solver.run(max_iters=1_000_000)
-    return solver.stats()
+    return _batch(solver, solves, np.full(len(solves), w))
 
 
 def _note(on_stats, st):

@@ -277,6 +277,14 @@ int fim_run(fim_ctx *c, int32_t max_iters, int32_t *iters);
  * (fim_seed / fim_seed_batch) while the others keep running: each map comes
  * out bit-identical to a solve started on its own. */
 int fim_advance(fim_ctx *c, int32_t iters, int32_t reset_stats, int32_t *left);
+/* fim_run for n distinct converge-mode contexts at once (no counterpart in
+ * the reference): one host thread keeps every context's stream fed, so the
+ * contexts' kernels share the device and one context's launch tail is
+ * filled by another's work (a batch split over two contexts of half the
+ * instances runs faster than one context; every map is the same as from
+ * fim_run).  iters (may be NULL) receives each context's global iterations;
+ * FIM_ENOCONV if any context reaches max_iters with work left. */
+int fim_run_multi(fim_ctx *const *cs, int32_t n, int32_t max_iters, int32_t *iters);
 int fim_active(fim_ctx *c, int32_t *active);
 int fim_recycle(fim_ctx *c, int32_t n, const int32_t *insts);
 /* global phi (min over copies; >= 1e50 means unreached, the reference uses

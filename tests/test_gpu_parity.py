@@ -287,6 +287,60 @@ def test_converge_work_units_and_grid_change_nothing(unit, grid):
         assert np.array_equal(a, b)
 
 
+def test_run_multi_contexts_change_nothing():
+    """12 instances at their own windows, solved by one context and by three
+    contexts of 5 + 4 + 3 instances run together (fim_run_multi, one host
+    thread feeding three streams): bitwise the same maps, so the oracle pins
+    of the single context carry over"""
+    cfg = cases.scaled(cases.SLAB, 8)
+    m, _ = cases.box(cfg, seed=9)
+    p = F.Pack(m, F.PART_RCB, max_nn=256, seed=9)
+    srcs = [cases.pacing_sites(cfg, 1 + k % 3, seed=300 + k) for k in range(12)]
+    wins = [250.0 + 75.0 * k for k in range(12)]
+    one = F.Solver(p, n_instances=12, window=500.0)
+    for i, src in enumerate(srcs):
+        one.set_window(wins[i], inst=i)
+        one.seed(src, 0.0, inst=i)
+    one.run()
+    base = [one.download(inst=i)[0] for i in range(12)]
+    iters_one = one.stats()["global_iters"]
+    one.close()
+    g = F.SolverGroup([F.Solver(p, n_instances=n, window=500.0) for n in (5, 4, 3)])
+    for ctx, a, b in g.parts(12):
+        for i in range(a, b):
+            ctx.set_window(wins[i], inst=i - a)
+            ctx.seed(srcs[i], 0.0, inst=i - a)
+    its = F.run_many(g.solvers)
+    assert len(its) == 3 and max(its) <= iters_one and all(k > 0 for k in its)
+    for i in range(12):
+        assert np.array_equal(g.download(inst=i)[0], base[i])
+    # a second run of the same group after a reset gives the same maps again
+    for ctx, a, b in g.parts(12):
+        ctx.reset()
+        for i in range(a, b):
+            ctx.seed(srcs[i], 0.0, inst=i - a)
+    g.run()
+    for i in range(12):
+        assert np.array_equal(g.download(inst=i)[0], base[i])
+    g.close()
+
+
+def test_run_multi_rejects_bad_context_lists():
+    cfg = cases.scaled(cases.SLAB, 8)
+    m, _ = cases.box(cfg, seed=9)
+    p = F.Pack(m, F.PART_RCB, max_nn=256, seed=9)
+    a = F.Solver(p, n_instances=1)
+    c = F.Solver(p, mode=F.MODE_COMPAT)
+    with pytest.raises(F.FimError, match="listed twice"):
+        F.run_many([a, a])
+    with pytest.raises(F.FimError, match="converge mode"):
+        F.run_many([a, c])
+    with pytest.raises(F.FimError, match="max_iters"):
+        F.run_many([a], max_iters=0)
+    a.close()
+    c.close()
+
+
 @pytest.mark.parametrize("staged", [False, True])
 @pytest.mark.parametrize("window", [300.0, 1000.0, 4000.0])
 def test_converge_time_window_same_fixed_point(window, staged):
