@@ -190,8 +190,9 @@ def main():
     ap.add_argument("--block", type=int, default=512)
     ap.add_argument("--local-iters", type=int, default=4,
                     help="sweeps per activation before requeueing (0 = to local convergence)")
-    ap.add_argument("--unit", type=int, default=2,
-                    help="queued instances of one submesh run back to back by one workgroup")
+    ap.add_argument("--unit", type=int, default=3,
+                    help="queued instances of one submesh run back to back by one workgroup "
+                         "(3 with two contexts; 2 is the best for one)")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--window", type=float, default=2000.0,
                     help="time window per global iteration in us: the cold batch's, and the base of "
