@@ -504,3 +504,32 @@ def test_scheduled_windows_and_vertex_sample():
         assert nearest in sample
     assert len(np.unique(sample)) == len(sample) <= 18
     assert np.array_equal(sample, WS.window_sample(m, k=10, seed=3))  # seeded
+
+
+def test_solver_group_maps_instances_to_contexts():
+    """SolverGroup's instance bookkeeping (host side only): instance i of the
+    group is instance i - off[k] of context k; parts() gives each context
+    its share of a batch of n solves"""
+    import fimpy as F
+
+    class Ctx:
+        def __init__(self, n):
+            self.n_instances, self.window, self.got = n, 2000.0, []
+
+        def download(self, inst=0):
+            self.got.append(inst)
+            return (inst, None)
+
+    a, b, c = Ctx(5), Ctx(4), Ctx(3)
+    g = F.SolverGroup([a, b, c])
+    assert g.n_instances == 12 and g.window == 2000.0
+    assert [(x is y, lo, hi) for (x, lo, hi), y in zip(g.parts(12), (a, b, c))] == \
+        [(True, 0, 5), (True, 5, 9), (True, 9, 12)]
+    assert [(lo, hi) for _, lo, hi in g.parts(7)] == [(0, 5), (5, 7)]
+    assert [(lo, hi) for _, lo, hi in g.parts(3)] == [(0, 3)]
+    for i in range(12):
+        g.download(inst=i)
+    assert a.got == [0, 1, 2, 3, 4] and b.got == [0, 1, 2, 3] and c.got == [0, 1, 2]
+    for bad in (-1, 12):
+        with pytest.raises(F.FimError):
+            g.download(inst=bad)
