@@ -323,6 +323,18 @@ def test_run_multi_contexts_change_nothing():
     for i in range(12):
         assert np.array_equal(g.download(inst=i)[0], base[i])
     g.close()
+    # profile mode (direct launches, every compute launch event-timed)
+    g = F.SolverGroup([F.Solver(p, n_instances=n, window=500.0, profile=True) for n in (6, 6)])
+    for ctx, a, b in g.parts(12):
+        for i in range(a, b):
+            ctx.set_window(wins[i], inst=i - a)
+            ctx.seed(srcs[i], 0.0, inst=i - a)
+    g.run()
+    st = g.stats()
+    assert st["compute_launches"] > 0 and st["compute_ms"] > 0.0
+    for i in range(12):
+        assert np.array_equal(g.download(inst=i)[0], base[i])
+    g.close()
 
 
 def test_run_multi_rejects_bad_context_lists():
