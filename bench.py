@@ -13,21 +13,22 @@ maps of the batched solver; that is set-up, outside the timed region.  One
 step = the 512 held-out solves as one batch: the surrogate predicts each
 solve's latest activation time (GPU inference over a vertex sample), every
 instance gets its own time window from it (fim_set_window), and the batched
-FIM runs to the eps = 1e-6 fixed point, iterations replayed from a captured
-hipGraph, mesh resident in HBM.  `value` is that rate; inference, seeding
+FIM runs to the eps = 1e-6 fixed point as two solver contexts of 256 run
+together (fim_run_multi: one context's launch tails filled by the other's
+work), iterations replayed from captured hipGraphs, mesh resident in HBM.  `value` is that rate; inference, seeding
 and reset are inside the timed region.
 
 Legs of the same run, in this order:
   surrogate    dataset (batched solves, every compute launch event-timed)
                and training;
-  accounting   one scheduled batch of the held-out solves with every compute
+  accounting   one scheduled batch of the held-out solves, in one context, with every compute
                launch timed by HIP events on the solver's stream (profile
                mode, the kernel sequence the graph replays): the
                k_conv_band dispatches right after the dataset's, from which
                the roofline comes;
   headline     W warm-up and K timed scheduled batches (graph replay);
   cold         the same solves at one uniform window (the method without
-               the surrogate), K timed batches;
+               the surrogate), same contexts, K timed batches;
   plain        plain single-solve GPU FIM on held-out solves (graph replay,
                its own best settings) -- the target's denominator;
   slab batch   the five-site slab maps of the earlier headline: a batch of
