@@ -349,6 +349,15 @@ def test_run_multi_rejects_bad_context_lists():
         F.run_many([a, c])
     with pytest.raises(F.FimError, match="max_iters"):
         F.run_many([a], max_iters=0)
+    # an iteration cap below what the solve needs ends every context's run
+    # and reports FIM_ENOCONV; a later run finishes the maps
+    b = F.Solver(p, n_instances=1, window=500.0)
+    b.seed(cases.pacing_sites(cfg, 1, seed=301), 0.0)
+    with pytest.raises(F.FimError, match="iteration cap"):
+        F.run_many([b], max_iters=1)
+    assert b.stats()["global_iters"] >= 1
+    F.run_many([b])
+    b.close()
     a.close()
     c.close()
 
