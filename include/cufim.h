@@ -246,7 +246,11 @@ int fim_seed(fim_ctx *c, int32_t inst, const int64_t *vtx, int64_t n, double t0)
  * for batched solves. */
 int fim_seed_batch(fim_ctx *c, int32_t n_inst, const int32_t *inst, const int64_t *off,
                    const int64_t *vtx, double t0);
-/* converge mode: per-instance, per-region scaling of both speeds (4 regions) */
+/* converge mode: per-instance, per-region scaling of both speeds (4 regions).
+ * fim_set_scaling and fim_set_window only record the value on the host; the
+ * values set since the last launch go to the device together before the
+ * next call that runs the instance (fim_run, fim_advance, fim_run_multi) or
+ * reads them (fim_residual, fim_support). */
 int fim_set_scaling(fim_ctx *c, int32_t inst, const float scale[4]);
 /* converge mode: the time window of one instance (us added per global
  * iteration, counted from the iteration it was seeded at; 0 = plain FIM
