@@ -1,6 +1,7 @@
 """Autotuner of the converge-mode solver: picks the submesh size, the
-workgroup shape, the local sweeps per global iteration, the time window and
-the compute kernel's register budget for a mesh and batch size by measuring
+workgroup shape, the local sweeps per global iteration, the time window, the
+compute kernel's register budget, the work-unit size and the number of
+contexts the batch is split over for a mesh and batch size by measuring
 short batched solves on the GPU.
 
 Search: coordinate descent from the defaults over
@@ -10,6 +11,9 @@ Search: coordinate descent from the defaults over
     window      {500, 1000, 2000, 4000}       microseconds per global iteration
     waves       {0, 5}                        waves per SIMD of the compute kernel (0 = LDS bound;
                                               workgroups of 256 only)
+    unit        {1, 2, 3, 4}                  queued instances of a submesh per work unit
+    contexts    {1, 2}                        contexts the batch is split over, run together
+                                              (fim_run_multi)
 Each trial times one batch solve after one warm-up solve.  Results are cached
 in a JSON file keyed by a mesh signature (sizes, bounding box, coordinate
 checksum) and the batch size.
@@ -23,7 +27,7 @@ import time
 
 import numpy as np
 
-from . import PART_RCB, Mesh, Pack, Solver
+from . import PART_RCB, Mesh, Pack, Solver, SolverGroup
 
 SPACE = {
     "max_nn": [256, 512, 1024, 1536, 2048, 2816],
@@ -31,8 +35,11 @@ SPACE = {
     "local_iters": [0, 1, 2, 4, 8],
     "window": [500.0, 1000.0, 2000.0, 4000.0],
     "waves": [0, 5],
+    "unit": [1, 2, 3, 4],
+    "contexts": [1, 2],
 }
-DEFAULT = {"max_nn": 2816, "block": 512, "local_iters": 4, "window": 2000.0, "waves": 0}
+DEFAULT = {"max_nn": 2816, "block": 512, "local_iters": 4, "window": 2000.0, "waves": 0, "unit": 2,
+           "contexts": 1}
 
 
 def signature(mesh: Mesh, n_instances: int) -> str:
@@ -50,15 +57,22 @@ def _measure(mesh, packs, cfg, seeds, scales, seed):
     key = cfg["max_nn"]
     if key not in packs:
         packs[key] = Pack(mesh, PART_RCB, max_nn=key, seed=seed)
-    s = Solver(packs[key], n_instances=len(seeds), block=cfg["block"],
-               local_iters=cfg["local_iters"], window=cfg["window"], waves=cfg.get("waves", 0))
+    n = len(seeds)
+    nctx = max(1, min(int(cfg.get("contexts", 1)), n))
+    cut = np.linspace(0, n, nctx + 1).astype(int)
+    ss = [Solver(packs[key], n_instances=int(cut[k + 1] - cut[k]), block=cfg["block"],
+                 local_iters=cfg["local_iters"], window=cfg["window"], waves=cfg.get("waves", 0),
+                 unit=cfg.get("unit", 0)) for k in range(nctx)]
+    s = ss[0] if nctx == 1 else SolverGroup(ss)
+    parts = [(ss[0], 0, n)] if nctx == 1 else s.parts(n)
     try:
         def solve():
-            s.reset()
-            if scales is not None:
-                for i in range(len(seeds)):
-                    s.set_scaling(scales[i], inst=i)
-            s.seed_batch(seeds, 0.0)
+            for ctx, a, b in parts:
+                ctx.reset()
+                if scales is not None:
+                    for i in range(a, b):
+                        ctx.set_scaling(scales[i], inst=i - a)
+                ctx.seed_batch(seeds[a:b], 0.0)
             s.run(max_iters=1_000_000)
         solve()
         t0 = time.perf_counter()
@@ -103,7 +117,8 @@ def tune(mesh: Mesh, seeds, scales=None, space=None, cache=None, seed=0, rounds=
     best_v = run(best)
     for _ in range(rounds):
         improved = False
-        for k in [k for k in ("window", "max_nn", "block", "local_iters", "waves") if k in space]:
+        for k in [k for k in ("window", "max_nn", "block", "local_iters", "waves", "unit", "contexts")
+                  if k in space]:
             for v in space[k]:
                 cand = canonical(dict(best, **{k: v}))
                 r = run(cand)

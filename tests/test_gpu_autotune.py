@@ -22,7 +22,7 @@ def test_autotune_picks_measured_best_and_caches(tmp_path):
     seeds = [s["sites"] for s in solves]
     scales = np.stack([s["scale"] for s in solves])
     space = {"max_nn": [256, 512], "block": [128, 256], "local_iters": [0, 2], "window": [1000.0],
-             "waves": [0, 5]}
+             "waves": [0, 5], "unit": [2, 3], "contexts": [1, 2]}
     cache = str(tmp_path / "tune.json")
     best, trials = A.tune(m, seeds, scales, space=space, cache=cache, rounds=1)
     assert best["max_nn"] in space["max_nn"] and best["block"] in space["block"]

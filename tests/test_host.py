@@ -378,10 +378,12 @@ def test_autotune_search_logic_with_a_stand_in_timer(tmp_path, monkeypatch):
     import fimpy.autotune as A
     calls = []
 
-    def rate(cfg):  # optimum: 1536 copies, workgroups of 256 on 5 waves, 2 sweeps, 1 ms
+    def rate(cfg):  # optimum: 1536 copies, workgroups of 256 on 5 waves, 2 sweeps, 1 ms,
+        # units of 3, two contexts
         return (10000.0 - abs(cfg["max_nn"] - 1536) - 100 * abs(cfg["local_iters"] - 2)
                 - abs(cfg["window"] - 1000.0) + (50 if cfg["block"] == 256 else 0)
-                + (20 if cfg["block"] == 256 and cfg["waves"] == 5 else 0))
+                + (20 if cfg["block"] == 256 and cfg["waves"] == 5 else 0)
+                - 7 * abs(cfg["unit"] - 3) + (30 if cfg["contexts"] == 2 else 0))
 
     def fake_measure(mesh, packs, cfg, seeds, scales, seed):
         calls.append(dict(cfg))
@@ -391,7 +393,8 @@ def test_autotune_search_logic_with_a_stand_in_timer(tmp_path, monkeypatch):
     monkeypatch.setattr(A, "signature", lambda mesh, n: f"stand-in/{n}")
     cache = str(tmp_path / "tune.json")
     best, trials = A.tune(None, [[0]] * 8, cache=cache, rounds=3)
-    assert best == {"max_nn": 1536, "block": 256, "local_iters": 2, "window": 1000.0, "waves": 5}
+    assert best == {"max_nn": 1536, "block": 256, "local_iters": 2, "window": 1000.0, "waves": 5,
+                    "unit": 3, "contexts": 2}
     rates = {json.dumps(c, sort_keys=True): r for c, r in trials}
     assert rates[json.dumps(best, sort_keys=True)] == max(rates.values())
     assert all(c["waves"] == 0 for c in calls if c["block"] != 256)
